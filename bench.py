@@ -1,0 +1,157 @@
+"""Benchmark: HistoPyramid marching cubes of a 512^3 FP32 volume on one MI355X.
+
+Workload: the Cayley cubic f = 4(x^2+y^2+z^2) + 16xyz - 1, x = 2p - 1 at
+texel centres, materialised once into a 512^3 FP32 volume (512 MiB) resident
+in HBM, iso-value 0: 133,432,831 cells, 3,694,104 vertices.  One step is one
+HPMCbuildHistopyramid (classify + pyramid reduction) plus one
+HPMCextractVertices writing positions and normals (N3F_V3F) for every vertex;
+no host synchronisation inside a step.
+
+Prints one JSON line.  value = cells/s over the whole step (the reference's
+own "mvps" definition, apps/texture3d/texture3d.cpp:279-288); the line also
+carries verts/s, the whole-step HBM fraction for the algorithmic bytes
+B = 4 * lattice points + 24 * vertices, per-stage HIP-event times, the
+roofline of the dominant kernel and the CPU baseline (the oracle's C port of
+the same path, one thread, on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cells/s + verts/s at 512^3 fp32, % HBM peak; batched/plain speedup"
+HBM_PEAK_GBS = 8000.0
+N_CAYLEY_512 = 3694104
+
+
+def cayley_volume_torch(n, device):
+    """The Cayley field in FP32 with the same operation order as
+    hpmc_amd.workloads.cayley (elementwise IEEE ops, bit-identical)."""
+    import torch
+    inv = torch.tensor(1.0, dtype=torch.float32) / torch.tensor(float(n), dtype=torch.float32)
+    p = (torch.arange(n, dtype=torch.float32, device=device) + 0.5) * inv.to(device)
+    x = (2.0 * p - 1.0).view(1, 1, n)
+    y = (2.0 * p - 1.0).view(1, n, 1)
+    z = (2.0 * p - 1.0).view(n, 1, 1)
+    s = (x * x + y * y) + z * z
+    xyz = ((16.0 * x) * y) * z
+    return (4.0 * s + xyz - 1.0).contiguous()
+
+
+def cpu_baseline(n, reps):
+    """Oracle C port of classify -> ordered list -> FP32 emission, one thread,
+    over the whole 512^3 workload, median of `reps` runs."""
+    import oracle
+    from hpmc_amd import workloads
+    f = workloads.cayley((n, n, n))
+    times = []
+    nv = 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        nv = oracle.run_volume(f, 0.0)
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[len(times) // 2]
+    cells = (n - 1) ** 3
+    return {"value": cells / t, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": "full %d^3 Cayley volume, %d runs (median %.2f s), %d vertices" % (n, reps, t, nv)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the engine builds one volume per GPU with no "
+              "collectives; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import hpmc_amd
+    n = args.size
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    vol = cayley_volume_torch(n, dev)
+    iso = hpmc_amd.IsoSurface((n, n, n))
+    iso.set_volume(vol)
+    iso.build(0.0)
+    nv = iso.count()
+    if n == 512:
+        assert nv == N_CAYLEY_512, nv
+    out = torch.empty((nv, 6), dtype=torch.float32, device=dev)
+    iso.bind_output(out)
+
+    for _ in range(args.warmup):
+        iso.build(0.0)
+        iso.extract()
+    torch.cuda.synchronize()
+
+    # timed region, stage events on the library's stream (the null stream)
+    iso.stage_times()
+    iso.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        iso.build(0.0)
+        iso.extract()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    iso.set_profiling(False)
+    st = iso.stage_times()
+    # wall time of the same steps without events between the kernels
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        iso.build(0.0)
+        iso.extract()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    assert iso.count() == nv
+
+    ms_step = (t3 - t2) * 1e3 / args.steps
+    cells = (n - 1) ** 3
+    lattice_bytes = 4 * n ** 3
+    algo_bytes = lattice_bytes + 24 * nv
+    stage_ms = {k: v[0] / max(v[1], 1) for k, v in st.items()}
+    cls_ms = stage_ms["classify"]
+    achieved = lattice_bytes / (cls_ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": cells / (ms_step * 1e-3),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Cayley cubic materialised as a %d^3 FP32 volume)" % n,
+        "config": {"workload": "cayley-%d^3-fp32-volume build+extract, iso 0, positions+normals" % n,
+                   "lattice": [n, n, n], "cells": cells, "vertices": nv, "iso": 0.0},
+        "verts_per_s": nv / (ms_step * 1e-3),
+        "hbm_frac_step": algo_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "algorithmic_bytes_step": algo_bytes,
+        "stage_ms": stage_ms,
+        "ms_per_step_with_stage_events": (t1 - t0) * 1e3 / args.steps,
+        "roofline": {"kernel": "hpmc_classify_volume", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(n, args.cpu_reps)
+    print(json.dumps(line))
+    iso.close()
+
+
+if __name__ == "__main__":
+    main()

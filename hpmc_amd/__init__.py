@@ -1,0 +1,265 @@
+"""MI355X HistoPyramid marching cubes: Python mirror of HPMC's C API.
+
+``hpmc_amd.lib()`` is the ctypes handle on ``libhpmc_amd.so`` (the C ABI of
+``include/hpmc.h``); every ``HPMC*`` function of the reference's public API
+(hpmc/include/hpmc.h:295-544) is bound under its own name with the same
+arguments, GL object names replaced by device pointers.  ``IsoSurface`` wraps
+the usual call sequence (constants -> HistoPyramid -> setters -> build ->
+count -> traversal handle -> extract) around torch device tensors.
+
+The library is HIP-only: importing works anywhere, but any call that needs
+the library raises ``RuntimeError`` when ``libhpmc_amd.so`` is missing, and
+device calls fail (no fallback path exists).
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhpmc_amd.so")
+
+HPMC_OK = 0
+ERRORS = {
+    0: "HPMC_OK", 1: "HPMC_ERR_NULL", 2: "HPMC_ERR_BROKEN", 3: "HPMC_ERR_SIZE",
+    4: "HPMC_ERR_FIELD", 5: "HPMC_ERR_COMPILE", 6: "HPMC_ERR_HIP",
+    7: "HPMC_ERR_CAPACITY", 8: "HPMC_ERR_ARG", 9: "HPMC_ERR_UNSUPPORTED",
+}
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_u8 = ctypes.c_uint8
+_f = ctypes.c_float
+_b = ctypes.c_bool
+
+# (name, restype, argtypes) of every export of include/hpmc.h
+SIGNATURES = [
+    ("HPMCcreateConstants", _vp, [_i32, _i32]),
+    ("HPMCdestroyConstants", None, [_vp]),
+    ("HPMCcreateHistoPyramid", _vp, [_vp]),
+    ("HPMCsetLatticeSize", None, [_vp, _i32, _i32, _i32]),
+    ("HPMCsetFieldAsBinary", None, [_vp]),
+    ("HPMCsetFieldAsContinuous", None, [_vp]),
+    ("HPMCsetGridSize", None, [_vp, _i32, _i32, _i32]),
+    ("HPMCsetGridExtent", None, [_vp, _f, _f, _f]),
+    ("HPMCsetFieldTexture3D", None, [_vp, _vp, _u8]),
+    ("HPMCsetFieldCustom", None, [_vp, ctypes.c_char_p, _u32, _u8]),
+    ("HPMCgetBuilderProgram", _u32, [_vp]),
+    ("HPMCdestroyHandle", None, [_vp]),
+    ("HPMCbuildHistopyramid", None, [_vp, _f]),
+    ("HPMCacquireNumberOfVertices", _u32, [_vp]),
+    ("HPMCcreateTraversalHandle", _vp, [_vp]),
+    ("HPMCdestroyTraversalHandle", None, [_vp]),
+    ("HPMCgetTraversalShaderFunctions", _vp, [_vp]),
+    ("HPMCsetTraversalHandleProgram", _b, [_vp, _u32, _u32, _u32, _u32]),
+    ("HPMCextractVertices", _b, [_vp]),
+    ("HPMCextractVerticesTransformFeedback", _b, [_vp]),
+    ("HPMCextractVerticesTransformFeedbackNV", _b, [_vp]),
+    ("HPMCextractVerticesTransformFeedbackEXT", _b, [_vp]),
+    ("HPMCgetLastError", _i32, [_vp]),
+    ("HPMCgetLastErrorString", ctypes.c_char_p, [_vp]),
+    ("HPMCsetStream", _i32, [_vp, _vp]),
+    ("HPMCsetFieldUniform", _i32, [_vp, ctypes.c_char_p, _vp, ctypes.c_size_t]),
+    ("HPMCsetTraversalHandleBuffer", _i32, [_vp, _vp, _u64]),
+    ("HPMCsetTraversalHandleCellEdgeBuffer", _i32, [_vp, _vp, _u64]),
+    ("HPMCacquireDroppedVertices", _u32, [_vp]),
+    ("HPMCgetCellCodes", _i32, [_vp, _vp]),
+    ("HPMCgetLevel4", _u64, [_vp, _vp, _u64]),
+    ("HPMCsetProfiling", _i32, [_vp, _i32]),
+    ("HPMCgetStageTimes", _i32, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u32)]),
+    ("HPMCcheckFieldSnippet", _i32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
+    ("HPMCgetLayout", _i32, [_vp, ctypes.POINTER(_i32)]),
+]
+
+_LIB = None
+
+
+def lib():
+    """The loaded C library; raises RuntimeError if it was never built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libhpmc_amd.so is missing: run `python -m hpmc_amd.build` "
+                               "(the engine has no non-HIP fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def last_error(h=None):
+    L = lib()
+    return int(L.HPMCgetLastError(h)), (L.HPMCgetLastErrorString(h) or b"").decode()
+
+
+class HPMCError(RuntimeError):
+    pass
+
+
+def _check(code, h=None):
+    if code != HPMC_OK:
+        err, msg = last_error(h)
+        raise HPMCError("%s: %s" % (ERRORS.get(code, code), msg))
+
+
+def check_snippet(source):
+    """Compile a field snippet for gfx950 without a device; returns the log."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    code = lib().HPMCcheckFieldSnippet(source.encode(), buf, len(buf))
+    _check(code)
+    return buf.value.decode()
+
+
+def _stream_ptr(stream):
+    return ctypes.c_void_p(stream.cuda_stream) if stream is not None else None
+
+
+class IsoSurface:
+    """One HistoPyramid plus one traversal handle, on torch's current device.
+
+    Mirrors the reference call order (apps/texture3d/texture3d.cpp:144-221):
+    create constants and HP, set lattice/grid/extent and field, build with an
+    iso-value, count, extract into an N3F_V3F buffer.
+    """
+
+    _constants = None
+
+    def __init__(self, lattice, grid=None, extent=(1.0, 1.0, 1.0), stream=None):
+        L = lib()
+        if IsoSurface._constants is None:
+            c = L.HPMCcreateConstants(4, 3)
+            if not c:
+                raise HPMCError("HPMCcreateConstants failed: %s" % (last_error()[1],))
+            IsoSurface._constants = c
+        self.h = L.HPMCcreateHistoPyramid(IsoSurface._constants)
+        if not self.h:
+            raise HPMCError("HPMCcreateHistoPyramid failed")
+        self.lattice = tuple(int(v) for v in lattice)
+        L.HPMCsetLatticeSize(self.h, *self.lattice)
+        self.grid = tuple(int(v) for v in grid) if grid is not None else tuple(v - 1 for v in self.lattice)
+        if grid is not None:
+            L.HPMCsetGridSize(self.h, *self.grid)
+        L.HPMCsetGridExtent(self.h, *[float(v) for v in extent])
+        if stream is not None:
+            _check(L.HPMCsetStream(self.h, _stream_ptr(stream)), self.h)
+        self.th = None
+        self._volume = None
+        self._out = None
+        self._ce = None
+
+    # -- field ---------------------------------------------------------------
+    def set_volume(self, volume):
+        """volume: contiguous float32 device tensor shaped (z, y, x)."""
+        import torch
+        assert volume.dtype == torch.float32 and volume.is_contiguous() and volume.is_cuda
+        assert tuple(volume.shape) == self.lattice[::-1], (volume.shape, self.lattice)
+        self._volume = volume
+        lib().HPMCsetFieldTexture3D(self.h, ctypes.c_void_p(volume.data_ptr()), 0)
+
+    def set_snippet(self, source):
+        lib().HPMCsetFieldCustom(self.h, source.encode(), 0, 0)
+
+    def set_uniform(self, name, array):
+        import numpy as np
+        a = np.ascontiguousarray(array)
+        _check(lib().HPMCsetFieldUniform(self.h, name.encode(), a.ctypes.data_as(ctypes.c_void_p), a.nbytes),
+               self.h)
+
+    def layout(self):
+        out = (_i32 * 5)()
+        _check(lib().HPMCgetLayout(self.h, out), self.h)
+        return tuple(out)
+
+    # -- build / count ----------------------------------------------------------
+    def build(self, iso):
+        L = lib()
+        L.HPMCbuildHistopyramid(self.h, float(iso))
+        err, msg = last_error(self.h)
+        if err != HPMC_OK:
+            raise HPMCError("build failed: %s: %s" % (ERRORS.get(err, err), msg))
+
+    def count(self):
+        return int(lib().HPMCacquireNumberOfVertices(self.h))
+
+    # -- extraction ----------------------------------------------------------
+    def _traversal(self):
+        if self.th is None:
+            self.th = lib().HPMCcreateTraversalHandle(self.h)
+            if not self.th:
+                raise HPMCError("HPMCcreateTraversalHandle failed: %s" % (last_error(self.h)[1],))
+        return self.th
+
+    def bind_output(self, out, cell_edge=None):
+        """out: float32 device tensor (capacity, 6); cell_edge: int32 (capacity, 2) or None."""
+        L = lib()
+        th = self._traversal()
+        self._out = out
+        _check(L.HPMCsetTraversalHandleBuffer(th, ctypes.c_void_p(out.data_ptr()), out.shape[0]), self.h)
+        self._ce = cell_edge
+        if cell_edge is not None:
+            _check(L.HPMCsetTraversalHandleCellEdgeBuffer(th, ctypes.c_void_p(cell_edge.data_ptr()),
+                                                          cell_edge.shape[0]), self.h)
+        else:
+            _check(L.HPMCsetTraversalHandleCellEdgeBuffer(th, None, 0), self.h)
+
+    def extract(self):
+        if not lib().HPMCextractVertices(self._traversal()):
+            raise HPMCError("extract failed: %s" % (last_error(self.h)[1],))
+
+    def extract_all(self, iso, with_cell_edge=False):
+        """Build, count and extract into fresh tensors: (vertices (N,6), cell_edge (N,2) or None)."""
+        import torch
+        self.build(iso)
+        n = self.count()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        out = torch.empty((max(n, 1), 6), dtype=torch.float32, device=dev)
+        ce = torch.empty((max(n, 1), 2), dtype=torch.int32, device=dev) if with_cell_edge else None
+        self.bind_output(out, ce)
+        if n:
+            self.extract()
+        return out[:n], (ce[:n] if ce is not None else None)
+
+    def cell_codes(self):
+        import torch
+        n = self.grid[0] * self.grid[1] * self.grid[2]
+        codes = torch.empty(n, dtype=torch.uint8, device="cuda")
+        _check(lib().HPMCgetCellCodes(self.h, ctypes.c_void_p(codes.data_ptr())), self.h)
+        return codes.view(self.grid[2], self.grid[1], self.grid[0])
+
+    def level4(self):
+        import torch
+        lay = self.layout()
+        side = (2 << lay[4]) // 16
+        buf = torch.empty(side * side, dtype=torch.int32, device="cuda")
+        n = lib().HPMCgetLevel4(self.h, ctypes.c_void_p(buf.data_ptr()), buf.numel())
+        if n == 0:
+            raise HPMCError("level4 failed: %s" % (last_error(self.h)[1],))
+        return buf
+
+    def set_profiling(self, enable=True):
+        _check(lib().HPMCsetProfiling(self.h, int(bool(enable))), self.h)
+
+    def stage_times(self):
+        """{stage: (total ms, launches)} since the last call (waits for the device)."""
+        ms = (ctypes.c_double * 3)()
+        n = (_u32 * 3)()
+        _check(lib().HPMCgetStageTimes(self.h, ms, n), self.h)
+        return {name: (ms[i], n[i]) for i, name in enumerate(("classify", "scan", "emit"))}
+
+    def close(self):
+        L = lib()
+        if self.th:
+            L.HPMCdestroyTraversalHandle(self.th)
+            self.th = None
+        if self.h:
+            L.HPMCdestroyHandle(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,898 @@
+// Host side of the MI355X HistoPyramid marching-cubes engine: the C ABI
+// declared in include/hpmc.h, state machine, layout, kernel launches and the
+// hipRTC path for field snippets.
+//
+// Reference behaviour mirrored here (paths relative to hetcomp/hpmc):
+//   handle life cycle, taint/broken latch ...... hpmc/src/hpiface.cpp:45-200
+//   lazy setup on build ........................ hpmc/src/init.cpp:48-69
+//   layout (tiles of z-slices in a 2^l2 square)  hpmc/src/init.cpp:72-149
+//   build / count .............................. hpmc/src/hpiface.cpp:225-345
+//   traversal handle and extraction ............ hpmc/src/triface.cpp:43-500
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hpmc.h"
+#include "hpmc_device.h"
+
+// Device sources embedded for hipRTC (generated by hpmc_amd/build.py).
+#include "_embedded_sources.inc"
+
+HPMC_DEFINE_FIELD_KERNELS(volume, (hpmc::VolumeField{(const float*)fsrc, g.sx, g.sy, g.sz}))
+
+extern "C" __global__ __launch_bounds__(256) void hpmc_scan_reduce(const uint32_t* level4, uint32_t n,
+                                                                   uint2* tile_sums)
+{
+    hpmc::scan_reduce_body(level4, n, tile_sums);
+}
+
+extern "C" __global__ __launch_bounds__(256) void hpmc_scan_scatter(const uint32_t* level4, uint32_t n,
+                                                                    const uint2* tile_sums, uint32_t ntiles,
+                                                                    uint32_t* act_key, uint32_t* act_off,
+                                                                    hpmc::Totals* totals)
+{
+    hpmc::scan_scatter_body(level4, n, tile_sums, ntiles, act_key, act_off, totals);
+}
+
+namespace {
+
+const uint64_t kCases[256] = HPMC_MC_CASES_INIT;
+
+enum FieldMode { kFieldVolume = 0, kFieldCustom = 1 };
+
+thread_local int32_t t_last_error = HPMC_OK;
+thread_local std::string t_last_msg;
+
+}  // namespace
+
+struct HPMCConstants {
+    int device = 0;
+    uint64_t* d_cases = nullptr;
+};
+
+struct HPMCHistoPyramid {
+    HPMCConstants* constants = nullptr;
+    bool tainted = true;
+    bool broken = true;
+    int32_t size[3] = {0, 0, 0};
+    int32_t cells[3] = {0, 0, 0};
+    float extent[3] = {1.0f, 1.0f, 1.0f};
+    bool binary = false;
+    int mode = kFieldVolume;
+    const float* volume = nullptr;
+    bool gradient = false;
+    std::string snippet;
+    // layout (init.cpp:105-130)
+    int32_t tile[2] = {0, 0};
+    int32_t tiles[2] = {0, 0};
+    int32_t l2 = 0;
+    hpmc::Geom geom{};
+    int zseg = 16;
+    // device state
+    uint32_t* level4 = nullptr;
+    uint64_t n_level4 = 0;
+    uint2* tile_sums = nullptr;
+    uint32_t n_tiles = 0;
+    uint32_t* act_key = nullptr;
+    uint32_t* act_off = nullptr;
+    hpmc::Totals* d_totals = nullptr;
+    hpmc::Totals* h_totals = nullptr;  // pinned
+    hipEvent_t ev_totals = nullptr;
+    bool built = false;
+    bool count_valid = false;
+    uint32_t count = 0;
+    float threshold = 0.0f;
+    hipStream_t stream = nullptr;
+    // hipRTC module for snippets
+    hipModule_t module = nullptr;
+    hipFunction_t k_classify = nullptr, k_emit = nullptr, k_codes = nullptr;
+    uint32_t builder_id = 0;
+    // optional per-stage timing: event pairs recorded around each launch
+    bool profiling = false;
+    std::vector<hipEvent_t> stage_events[3];
+    // errors
+    int32_t last_error = HPMC_OK;
+    std::string last_msg;
+};
+
+struct HPMCTraversalHandle {
+    HPMCHistoPyramid* h = nullptr;
+    uint32_t program = 0;
+    float* out = nullptr;
+    uint64_t capacity = 0;
+    hpmc::CellEdge* out_ce = nullptr;
+    uint64_t ce_capacity = 0;
+};
+
+namespace {
+
+int32_t fail(HPMCHistoPyramid* h, int32_t code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_last_error = code;
+    t_last_msg = buf;
+    if (h) {
+        h->last_error = code;
+        h->last_msg = buf;
+    }
+    return code;
+}
+
+void ok(HPMCHistoPyramid* h)
+{
+    t_last_error = HPMC_OK;
+    t_last_msg.clear();
+    if (h) {
+        h->last_error = HPMC_OK;
+        h->last_msg.clear();
+    }
+}
+
+#define HIP_TRY(h, expr)                                                                   \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess) {                                                            \
+            fail((h), HPMC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e));               \
+            return false;                                                                  \
+        }                                                                                  \
+    } while (0)
+
+void free_device(HPMCHistoPyramid* h)
+{
+    if (h->level4) (void)hipFree(h->level4);
+    if (h->tile_sums) (void)hipFree(h->tile_sums);
+    if (h->act_key) (void)hipFree(h->act_key);
+    if (h->act_off) (void)hipFree(h->act_off);
+    if (h->d_totals) (void)hipFree(h->d_totals);
+    if (h->h_totals) (void)hipHostFree(h->h_totals);
+    if (h->ev_totals) (void)hipEventDestroy(h->ev_totals);
+    if (h->module) (void)hipModuleUnload(h->module);
+    h->level4 = nullptr;
+    h->tile_sums = nullptr;
+    h->act_key = h->act_off = nullptr;
+    h->d_totals = nullptr;
+    h->h_totals = nullptr;
+    h->ev_totals = nullptr;
+    h->module = nullptr;
+    h->k_classify = h->k_emit = h->k_codes = nullptr;
+    h->built = false;
+    h->count_valid = false;
+}
+
+// hpmc/src/init.cpp:72-149.  FP32 log2f/ceilf/sqrtf as in the reference.
+bool determine_layout(HPMCHistoPyramid* h)
+{
+    for (int a = 0; a < 3; ++a) {
+        if (h->size[a] < 2 || h->size[a] > 16384) {
+            fail(h, HPMC_ERR_SIZE, "lattice size %d out of [2,16384] along axis %d", h->size[a], a);
+            return false;
+        }
+        if (h->size[a] <= h->cells[a] || h->cells[a] < 1) {
+            fail(h, HPMC_ERR_SIZE, "grid size %d must be in [1, lattice-1=%d] along axis %d", h->cells[a],
+                 h->size[a] - 1, a);
+            return false;
+        }
+    }
+    // cells == 1 along x or y gives log2f(0.5f) = -1 and a negative shift in
+    // the reference (init.cpp:106-111); it cannot lay out such a grid either.
+    if (h->cells[0] < 2 || h->cells[1] < 2) {
+        fail(h, HPMC_ERR_SIZE, "grid must have at least 2 cells along x and y");
+        return false;
+    }
+    int32_t ts0 = 1 << (int32_t)ceilf(log2f((float)h->cells[0] / 2.0f));
+    int32_t ts1 = 1 << (int32_t)ceilf(log2f((float)h->cells[1] / 2.0f));
+    float aspect = (float)ts0 / (float)ts1;
+    float l = ceilf(log2f(sqrtf((float)h->cells[2] / aspect)));
+    if (l < 0.0f) l = 0.0f;
+    int32_t lay0 = 1 << (int32_t)l;
+    int32_t lay1 = (h->cells[2] + lay0 - 1) / lay0;
+    int64_t w0 = (int64_t)ts0 * lay0, w1 = (int64_t)ts1 * lay1;
+    int32_t l2 = (int32_t)ceilf(log2f((float)(w0 > w1 ? w0 : w1)));
+    int32_t S = 1 << l2;
+    h->tile[0] = ts0;
+    h->tile[1] = ts1;
+    h->tiles[0] = S / ts0;
+    h->tiles[1] = S / ts1;
+    h->l2 = l2;
+    return true;
+}
+
+std::string rtc_source(const std::string& snippet)
+{
+    std::string s;
+    s += "#define HPMC_RTC 1\n";
+    s += "typedef unsigned char uint8_t;\ntypedef unsigned short uint16_t;\n";
+    s += "typedef unsigned int uint32_t;\ntypedef unsigned long long uint64_t;\n";
+    s += "#include \"hpmc_device.h\"\n";
+    s += "#line 1 \"field_snippet\"\n";
+    s += snippet;
+    s += "\n#line 1 \"hpmc_rtc_tail\"\n";
+    s += "namespace hpmc {\n"
+         "struct SnippetField {\n"
+         "    float ipx, ipy, ipz;\n"
+         "    __device__ __forceinline__ float at(int i, int j, int k) const {\n"
+         "        return HPMC_fetch(make_float3(((float)i + 0.5f) * ipx, ((float)j + 0.5f) * ipy,\n"
+         "                                      ((float)k + 0.5f) * ipz));\n"
+         "    }\n"
+         "    __device__ __forceinline__ float row(int i, int j, int k) const { return at(i, j, k); }\n"
+         "};\n"
+         "}\n"
+         "HPMC_DEFINE_FIELD_KERNELS(rtc, (hpmc::SnippetField{g.ipx, g.ipy, g.ipz}))\n";
+    return s;
+}
+
+std::mutex g_rtc_mutex;
+uint32_t g_builder_ids = 0;
+
+// Compiles the snippet behind the device header; returns the code object
+// or an empty vector with the compiler log in *log.  Needs no device.
+std::vector<char> rtc_compile(const std::string& snippet, std::string* log)
+{
+    std::lock_guard<std::mutex> lock(g_rtc_mutex);
+    std::string src = rtc_source(snippet);
+    const char* hdr_src[2] = {kEmbeddedDeviceHeader, kEmbeddedTablesHeader};
+    const char* hdr_names[2] = {"hpmc_device.h", "mc_tables.h"};
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "hpmc_field.hip", 2, hdr_src, hdr_names) != HIPRTC_SUCCESS) {
+        *log = "hiprtcCreateProgram failed";
+        return {};
+    }
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    std::string text(n + 1, '\0');
+    hiprtcGetProgramLog(prog, &text[0]);
+    text.resize(strlen(text.c_str()));
+    *log = text;
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return {};
+    }
+    size_t code_size = 0;
+    hiprtcGetCodeSize(prog, &code_size);
+    std::vector<char> code(code_size);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return code;
+}
+
+bool compile_snippet(HPMCHistoPyramid* h)
+{
+    std::string log;
+    std::vector<char> code = rtc_compile(h->snippet, &log);
+    if (code.empty()) {
+        fail(h, HPMC_ERR_COMPILE, "field snippet failed to compile: %s", log.c_str());
+        return false;
+    }
+    HIP_TRY(h, hipModuleLoadData(&h->module, code.data()));
+    HIP_TRY(h, hipModuleGetFunction(&h->k_classify, h->module, "hpmc_classify_rtc"));
+    HIP_TRY(h, hipModuleGetFunction(&h->k_emit, h->module, "hpmc_emit_rtc"));
+    HIP_TRY(h, hipModuleGetFunction(&h->k_codes, h->module, "hpmc_codes_rtc"));
+    return true;
+}
+
+int choose_zseg(const hpmc::Geom& g)
+{
+    const int64_t per_slice = (int64_t)((g.cx + 63) / 64) * ((g.cy + hpmc::kBlock - 1) / hpmc::kBlock);
+    const int64_t target_waves = 8192;
+    int64_t z = (g.cz * per_slice + target_waves - 1) / target_waves;
+    if (z < 8) z = 8;
+    if (z > 64) z = 64;
+    return (int)z;
+}
+
+// hpmc/src/init.cpp:48-69: layout, storage and builder programs, once per
+// configuration.
+bool setup(HPMCHistoPyramid* h)
+{
+    if (!h->tainted) return true;
+    free_device(h);
+    if (!determine_layout(h)) return false;
+    const int32_t tsx2 = 2 * h->tile[0], tsy2 = 2 * h->tile[1];
+    if (tsx2 < hpmc::kBlock || tsy2 < hpmc::kBlock) {
+        fail(h, HPMC_ERR_UNSUPPORTED, "grids with 8 or fewer cells along x or y are not supported");
+        return false;
+    }
+    if (h->mode == kFieldVolume && !h->volume) {
+        fail(h, HPMC_ERR_FIELD, "no field volume set");
+        return false;
+    }
+    if (h->mode == kFieldCustom && h->snippet.empty()) {
+        fail(h, HPMC_ERR_FIELD, "empty field snippet");
+        return false;
+    }
+    if (h->gradient) {
+        fail(h, HPMC_ERR_UNSUPPORTED, "gradient-providing fields are not supported yet");
+        return false;
+    }
+    if (h->binary) {
+        fail(h, HPMC_ERR_UNSUPPORTED, "binary fields are not supported yet");
+        return false;
+    }
+    hpmc::Geom& g = h->geom;
+    g.sx = h->size[0];
+    g.sy = h->size[1];
+    g.sz = h->size[2];
+    g.cx = h->cells[0];
+    g.cy = h->cells[1];
+    g.cz = h->cells[2];
+    g.tsx2 = tsx2;
+    g.tsy2 = tsy2;
+    g.tiles_x = h->tiles[0];
+    g.l2 = h->l2;
+    g.blocks_log2 = h->l2 + 1 - hpmc::kBlockLog2;
+    g.scx = h->extent[0] / (float)h->cells[0];
+    g.scy = h->extent[1] / (float)h->cells[1];
+    g.scz = h->extent[2] / (float)h->cells[2];
+    g.ipx = 1.0f / (float)h->size[0];
+    g.ipy = 1.0f / (float)h->size[1];
+    g.ipz = 1.0f / (float)h->size[2];
+    h->zseg = choose_zseg(g);
+    const uint64_t side = (uint64_t)1 << g.blocks_log2;
+    h->n_level4 = side * side;
+    h->n_tiles = (uint32_t)((h->n_level4 + hpmc::kScanTile - 1) / hpmc::kScanTile);
+    HIP_TRY(h, hipMalloc(&h->level4, h->n_level4 * sizeof(uint32_t)));
+    HIP_TRY(h, hipMemset(h->level4, 0, h->n_level4 * sizeof(uint32_t)));
+    HIP_TRY(h, hipMalloc(&h->tile_sums, h->n_tiles * sizeof(uint2)));
+    HIP_TRY(h, hipMalloc(&h->act_key, h->n_level4 * sizeof(uint32_t)));
+    HIP_TRY(h, hipMalloc(&h->act_off, h->n_level4 * sizeof(uint32_t)));
+    HIP_TRY(h, hipMalloc(&h->d_totals, sizeof(hpmc::Totals)));
+    HIP_TRY(h, hipMemset(h->d_totals, 0, sizeof(hpmc::Totals)));
+    HIP_TRY(h, hipHostMalloc(&h->h_totals, sizeof(hpmc::Totals), hipHostMallocDefault));
+    memset(h->h_totals, 0, sizeof(hpmc::Totals));
+    HIP_TRY(h, hipEventCreateWithFlags(&h->ev_totals, hipEventDisableTiming));
+    HIP_TRY(h, hipDeviceSynchronize());
+    if (h->mode == kFieldCustom) {
+        if (!compile_snippet(h)) return false;
+    }
+    h->builder_id = ++g_builder_ids;
+    h->tainted = false;
+    return true;
+}
+
+enum Stage { kStageClassify = 0, kStageScan = 1, kStageEmit = 2 };
+
+void stage_mark(HPMCHistoPyramid* h, int stage)
+{
+    if (!h->profiling) return;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, h->stream);
+    h->stage_events[stage].push_back(e);
+}
+
+const void* field_source(HPMCHistoPyramid* h) { return h->mode == kFieldVolume ? (const void*)h->volume : nullptr; }
+
+bool launch_classify(HPMCHistoPyramid* h, float T)
+{
+    const hpmc::Geom& g = h->geom;
+    const int64_t waves = (int64_t)((g.cx + 63) / 64) * ((g.cy + hpmc::kBlock - 1) / hpmc::kBlock) *
+                          ((g.cz + h->zseg - 1) / h->zseg);
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    const void* fsrc = field_source(h);
+    stage_mark(h, kStageClassify);
+    if (h->mode == kFieldVolume) {
+        hipLaunchKernelGGL(hpmc_classify_volume, dim3(blocks), dim3(256), 0, h->stream, g, T, fsrc, h->level4,
+                           h->zseg, (const uint64_t*)h->constants->d_cases);
+    } else {
+        hpmc::Geom ga = g;
+        float Ta = T;
+        const void* fa = fsrc;
+        uint32_t* l4 = h->level4;
+        int zs = h->zseg;
+        const uint64_t* cs = h->constants->d_cases;
+        void* args[] = {&ga, &Ta, &fa, &l4, &zs, &cs};
+        HIP_TRY(h, hipModuleLaunchKernel(h->k_classify, blocks, 1, 1, 256, 1, 1, 0, h->stream, args, nullptr));
+    }
+    HIP_TRY(h, hipGetLastError());
+    stage_mark(h, kStageClassify);
+    return true;
+}
+
+bool launch_scan(HPMCHistoPyramid* h)
+{
+    const uint32_t n = (uint32_t)h->n_level4;
+    stage_mark(h, kStageScan);
+    hipLaunchKernelGGL(hpmc_scan_reduce, dim3(h->n_tiles), dim3(256), 0, h->stream, h->level4, n, h->tile_sums);
+    hipLaunchKernelGGL(hpmc_scan_scatter, dim3(h->n_tiles), dim3(256), 0, h->stream, h->level4, n,
+                       (const uint2*)h->tile_sums, h->n_tiles, h->act_key, h->act_off, h->d_totals);
+    HIP_TRY(h, hipGetLastError());
+    stage_mark(h, kStageScan);
+    return true;
+}
+
+int emit_grid(const HPMCHistoPyramid* h)
+{
+    // persistent: about eight 256-thread workgroups per CU, never more than
+    // there are blocks in the pyramid level.
+    uint64_t g = 2048;
+    if (h->n_level4 < g) g = h->n_level4;
+    return (int)(g ? g : 1);
+}
+
+bool launch_emit(HPMCHistoPyramid* h, float* out, hpmc::CellEdge* out_ce, uint32_t capacity)
+{
+    const hpmc::Geom& g = h->geom;
+    const int grid = emit_grid(h);
+    const void* fsrc = field_source(h);
+    stage_mark(h, kStageEmit);
+    if (h->mode == kFieldVolume) {
+        hipLaunchKernelGGL(hpmc_emit_volume, dim3(grid), dim3(256), 0, h->stream, g, h->threshold, fsrc,
+                           (const uint32_t*)h->act_key, (const uint32_t*)h->act_off, (const uint32_t*)h->level4,
+                           (const hpmc::Totals*)h->d_totals, out, out_ce, capacity,
+                           (const uint64_t*)h->constants->d_cases);
+    } else {
+        hpmc::Geom ga = g;
+        float Ta = h->threshold;
+        const void* fa = fsrc;
+        const uint32_t* ak = h->act_key;
+        const uint32_t* ao = h->act_off;
+        const uint32_t* l4 = h->level4;
+        const hpmc::Totals* tt = h->d_totals;
+        float* o = out;
+        hpmc::CellEdge* oc = out_ce;
+        uint32_t cap = capacity;
+        const uint64_t* cs = h->constants->d_cases;
+        void* args[] = {&ga, &Ta, &fa, &ak, &ao, &l4, &tt, &o, &oc, &cap, &cs};
+        HIP_TRY(h, hipModuleLaunchKernel(h->k_emit, grid, 1, 1, 256, 1, 1, 0, h->stream, args, nullptr));
+    }
+    HIP_TRY(h, hipGetLastError());
+    stage_mark(h, kStageEmit);
+    return true;
+}
+
+bool wait_count(HPMCHistoPyramid* h)
+{
+    if (h->count_valid) return true;
+    if (!h->built) {
+        h->count = 0;
+        return true;
+    }
+    HIP_TRY(h, hipEventSynchronize(h->ev_totals));
+    h->count = h->h_totals->vertices;
+    h->count_valid = true;
+    return true;
+}
+
+bool extract_helper(HPMCTraversalHandle* th)
+{
+    if (!th) {
+        fail(nullptr, HPMC_ERR_NULL, "NULL traversal handle");
+        return false;
+    }
+    HPMCHistoPyramid* h = th->h;
+    if (!h || h->broken) {
+        fail(h, HPMC_ERR_BROKEN, "HistoPyramid is broken");
+        return false;
+    }
+    if (h->tainted || !h->built) {
+        fail(h, HPMC_ERR_ARG, "HistoPyramid has not been built since it was configured");
+        return false;
+    }
+    if (!th->out) {
+        fail(h, HPMC_ERR_ARG, "no output buffer bound to the traversal handle");
+        return false;
+    }
+    uint64_t cap = th->capacity;
+    if (th->out_ce && th->ce_capacity < cap) cap = th->ce_capacity;
+    if (cap > 0xffffffffull) cap = 0xffffffffull;
+    if (h->count_valid && h->count > cap) {
+        fail(h, HPMC_ERR_CAPACITY, "output holds %llu vertices, the surface has %u", (unsigned long long)cap,
+             h->count);
+        return false;
+    }
+    if (!launch_emit(h, th->out, th->out_ce, (uint32_t)cap)) return false;
+    ok(h);
+    return true;
+}
+
+}  // namespace
+
+// =========================================================================
+extern "C" {
+
+struct HPMCConstants* HPMCcreateConstants(int32_t, int32_t)
+{
+    HPMCConstants* c = new (std::nothrow) HPMCConstants;
+    if (!c) return nullptr;
+    if (hipGetDevice(&c->device) != hipSuccess || hipMalloc(&c->d_cases, sizeof kCases) != hipSuccess ||
+        hipMemcpy(c->d_cases, kCases, sizeof kCases, hipMemcpyHostToDevice) != hipSuccess) {
+        fail(nullptr, HPMC_ERR_HIP, "failed to upload marching-cubes tables");
+        if (c->d_cases) (void)hipFree(c->d_cases);
+        delete c;
+        return nullptr;
+    }
+    ok(nullptr);
+    return c;
+}
+
+void HPMCdestroyConstants(struct HPMCConstants* c)
+{
+    if (!c) return;
+    if (c->d_cases) (void)hipFree(c->d_cases);
+    delete c;
+}
+
+struct HPMCHistoPyramid* HPMCcreateHistoPyramid(struct HPMCConstants* c)
+{
+    if (!c) {
+        fail(nullptr, HPMC_ERR_NULL, "createHistoPyramid called with constants = NULL");
+        return nullptr;
+    }
+    HPMCHistoPyramid* h = new (std::nothrow) HPMCHistoPyramid;
+    if (!h) return nullptr;
+    h->constants = c;
+    ok(nullptr);
+    return h;
+}
+
+void HPMCsetLatticeSize(struct HPMCHistoPyramid* h, int32_t x, int32_t y, int32_t z)
+{
+    if (!h) return;
+    h->size[0] = x;
+    h->size[1] = y;
+    h->size[2] = z;
+    for (int a = 0; a < 3; ++a) h->cells[a] = (h->size[a] > 1 ? h->size[a] : 1) - 1;
+    h->tainted = true;
+    h->broken = false;
+}
+
+void HPMCsetGridSize(struct HPMCHistoPyramid* h, int32_t x, int32_t y, int32_t z)
+{
+    if (!h) return;
+    h->cells[0] = x;
+    h->cells[1] = y;
+    h->cells[2] = z;
+    h->tainted = true;
+    h->broken = false;
+}
+
+void HPMCsetFieldAsBinary(struct HPMCHistoPyramid* h)
+{
+    if (h) h->binary = true;
+}
+
+void HPMCsetFieldAsContinuous(struct HPMCHistoPyramid* h)
+{
+    if (h) h->binary = false;
+}
+
+void HPMCsetGridExtent(struct HPMCHistoPyramid* h, float x, float y, float z)
+{
+    if (!h) return;
+    h->extent[0] = x;
+    h->extent[1] = y;
+    h->extent[2] = z;
+    h->tainted = true;
+    h->broken = false;
+}
+
+void HPMCsetFieldTexture3D(struct HPMCHistoPyramid* h, const float* volume, uint8_t gradient)
+{
+    if (!h) return;
+    // hpiface.cpp:164-184: a new texture alone does not force a rebuild of
+    // the programs; a change of mode or gradient does.
+    h->volume = volume;
+    const bool grad = gradient != 0;
+    if (h->mode != kFieldVolume || h->gradient != grad) {
+        h->mode = kFieldVolume;
+        h->gradient = grad;
+        h->tainted = true;
+        h->broken = false;
+    }
+}
+
+void HPMCsetFieldCustom(struct HPMCHistoPyramid* h, const char* source, uint32_t, uint8_t gradient)
+{
+    if (!h) return;
+    h->mode = kFieldCustom;
+    h->snippet = source ? source : "";
+    h->gradient = gradient != 0;
+    h->tainted = true;
+    h->broken = false;
+}
+
+uint32_t HPMCgetBuilderProgram(struct HPMCHistoPyramid* h)
+{
+    if (!h) {
+        fail(nullptr, HPMC_ERR_NULL, "h == NULL");
+        return 0;
+    }
+    if (h->broken) {
+        fail(h, HPMC_ERR_BROKEN, "h is broken");
+        return 0;
+    }
+    if (h->tainted && !setup(h)) return 0;
+    return h->builder_id;
+}
+
+void HPMCdestroyHandle(struct HPMCHistoPyramid* h)
+{
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    else (void)hipDeviceSynchronize();
+    free_device(h);
+    delete h;
+}
+
+void HPMCbuildHistopyramid(struct HPMCHistoPyramid* h, float threshold)
+{
+    if (!h || h->broken) return;
+    // hpiface.cpp:258-268: a failed setup leaves the HP tainted, not broken.
+    if (h->tainted && !setup(h)) return;
+    h->threshold = h->binary ? 0.5f : threshold;
+    h->count_valid = false;
+    h->built = false;
+    if (!launch_classify(h, h->threshold) || !launch_scan(h)) {
+        h->broken = true;
+        return;
+    }
+    if (hipMemcpyAsync(h->h_totals, h->d_totals, sizeof(hpmc::Totals), hipMemcpyDeviceToHost, h->stream) !=
+            hipSuccess ||
+        hipEventRecord(h->ev_totals, h->stream) != hipSuccess) {
+        fail(h, HPMC_ERR_HIP, "failed to enqueue the count readback");
+        h->broken = true;
+        return;
+    }
+    h->built = true;
+    ok(h);
+}
+
+uint32_t HPMCacquireNumberOfVertices(struct HPMCHistoPyramid* h)
+{
+    if (!h || h->broken) return 0;
+    if (!wait_count(h)) {
+        h->broken = true;
+        return 0;
+    }
+    return h->count;
+}
+
+struct HPMCTraversalHandle* HPMCcreateTraversalHandle(struct HPMCHistoPyramid* h)
+{
+    if (!h) {
+        fail(nullptr, HPMC_ERR_NULL, "createTraversalHandle called with h == NULL");
+        return nullptr;
+    }
+    if (!setup(h)) return nullptr;
+    HPMCTraversalHandle* th = new (std::nothrow) HPMCTraversalHandle;
+    if (!th) return nullptr;
+    th->h = h;
+    return th;
+}
+
+void HPMCdestroyTraversalHandle(struct HPMCTraversalHandle* th) { delete th; }
+
+char* HPMCgetTraversalShaderFunctions(struct HPMCTraversalHandle* th)
+{
+    if (!th || !th->h) {
+        fail(nullptr, HPMC_ERR_NULL, "getTraversalShaderFunctions called with th == NULL");
+        return nullptr;
+    }
+    HPMCHistoPyramid* h = th->h;
+    if (h->tainted && !setup(h)) return nullptr;
+    char defs[1024];
+    snprintf(defs, sizeof defs,
+             "// HPMC layout of this HistoPyramid (hpmc/src/shadergen.cpp:45-83)\n"
+             "#define HPMC_FUNC_X %d\n#define HPMC_FUNC_Y %d\n#define HPMC_FUNC_Z %d\n"
+             "#define HPMC_CELLS_X %d\n#define HPMC_CELLS_Y %d\n#define HPMC_CELLS_Z %d\n"
+             "#define HPMC_TILES_X %d\n#define HPMC_TILES_Y %d\n"
+             "#define HPMC_TILE_SIZE_X %d\n#define HPMC_TILE_SIZE_Y %d\n#define HPMC_HP_SIZE_L2 %d\n",
+             h->size[0], h->size[1], h->size[2], h->cells[0], h->cells[1], h->cells[2], h->tiles[0], h->tiles[1],
+             h->tile[0], h->tile[1], h->l2);
+    std::string s = defs;
+    s += kEmbeddedTablesHeader;
+    s += kEmbeddedDeviceHeader;
+    char* r = (char*)malloc(s.size() + 1);
+    if (!r) return nullptr;
+    memcpy(r, s.c_str(), s.size() + 1);
+    return r;
+}
+
+bool HPMCsetTraversalHandleProgram(struct HPMCTraversalHandle* th, uint32_t program, uint32_t u1, uint32_t u2,
+                                   uint32_t u3)
+{
+    if (!th) {
+        fail(nullptr, HPMC_ERR_NULL, "setTraversalHandleProgram called with th == NULL");
+        return false;
+    }
+    // triface.cpp:217-238: a program is required and the work units must be
+    // distinct (the third is unused by custom fetches).
+    if (program == 0) {
+        fail(th->h, HPMC_ERR_ARG, "program == 0");
+        return false;
+    }
+    const bool custom = th->h && th->h->mode == kFieldCustom;
+    if (u1 == u2 || (!custom && (u1 == u3 || u2 == u3))) {
+        fail(th->h, HPMC_ERR_ARG, "texture units must be distinct");
+        return false;
+    }
+    th->program = program;
+    return true;
+}
+
+bool HPMCextractVertices(struct HPMCTraversalHandle* th) { return extract_helper(th); }
+bool HPMCextractVerticesTransformFeedback(struct HPMCTraversalHandle* th) { return extract_helper(th); }
+bool HPMCextractVerticesTransformFeedbackNV(struct HPMCTraversalHandle* th) { return extract_helper(th); }
+bool HPMCextractVerticesTransformFeedbackEXT(struct HPMCTraversalHandle* th) { return extract_helper(th); }
+
+int32_t HPMCgetLastError(struct HPMCHistoPyramid* h) { return h ? h->last_error : t_last_error; }
+
+const char* HPMCgetLastErrorString(struct HPMCHistoPyramid* h) { return h ? h->last_msg.c_str() : t_last_msg.c_str(); }
+
+int32_t HPMCsetStream(struct HPMCHistoPyramid* h, void* stream)
+{
+    if (!h) return fail(nullptr, HPMC_ERR_NULL, "h == NULL");
+    h->stream = (hipStream_t)stream;
+    ok(h);
+    return HPMC_OK;
+}
+
+int32_t HPMCsetFieldUniform(struct HPMCHistoPyramid* h, const char* name, const void* data, size_t bytes)
+{
+    if (!h || !name || (!data && bytes)) return fail(h, HPMC_ERR_NULL, "NULL argument");
+    if (h->broken) return fail(h, HPMC_ERR_BROKEN, "h is broken");
+    if (h->mode != kFieldCustom) return fail(h, HPMC_ERR_ARG, "field is not a snippet");
+    if (h->tainted && !setup(h)) return h->last_error;
+    hipDeviceptr_t dptr = nullptr;
+    size_t sz = 0;
+    if (hipModuleGetGlobal(&dptr, &sz, h->module, name) != hipSuccess)
+        return fail(h, HPMC_ERR_ARG, "snippet has no __device__ global named '%s'", name);
+    if (bytes > sz) return fail(h, HPMC_ERR_ARG, "'%s' holds %zu bytes, got %zu", name, sz, bytes);
+    if (hipMemcpyHtoDAsync(dptr, const_cast<void*>(data), bytes, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return fail(h, HPMC_ERR_HIP, "failed to set '%s'", name);
+    ok(h);
+    return HPMC_OK;
+}
+
+int32_t HPMCsetTraversalHandleBuffer(struct HPMCTraversalHandle* th, float* out, uint64_t capacity)
+{
+    if (!th) return fail(nullptr, HPMC_ERR_NULL, "th == NULL");
+    th->out = out;
+    th->capacity = out ? capacity : 0;
+    ok(th->h);
+    return HPMC_OK;
+}
+
+int32_t HPMCsetTraversalHandleCellEdgeBuffer(struct HPMCTraversalHandle* th, void* records, uint64_t capacity)
+{
+    if (!th) return fail(nullptr, HPMC_ERR_NULL, "th == NULL");
+    th->out_ce = (hpmc::CellEdge*)records;
+    th->ce_capacity = records ? capacity : 0;
+    ok(th->h);
+    return HPMC_OK;
+}
+
+uint32_t HPMCacquireDroppedVertices(struct HPMCTraversalHandle* th)
+{
+    if (!th || !th->h || th->h->broken) return 0;
+    if (!wait_count(th->h)) return 0;
+    uint64_t cap = th->capacity;
+    if (th->out_ce && th->ce_capacity < cap) cap = th->ce_capacity;
+    return th->h->count > cap ? (uint32_t)(th->h->count - cap) : 0u;
+}
+
+int32_t HPMCgetCellCodes(struct HPMCHistoPyramid* h, uint8_t* codes)
+{
+    if (!h || !codes) return fail(h, HPMC_ERR_NULL, "NULL argument");
+    if (h->broken) return fail(h, HPMC_ERR_BROKEN, "h is broken");
+    if (h->tainted && !setup(h)) return h->last_error;
+    const hpmc::Geom& g = h->geom;
+    const uint64_t n = (uint64_t)g.cx * g.cy * g.cz;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 65536);
+    if (h->mode == kFieldVolume) {
+        hipLaunchKernelGGL(hpmc_codes_volume, dim3(blocks), dim3(256), 0, h->stream, g, h->threshold,
+                           field_source(h), codes);
+    } else {
+        hpmc::Geom ga = g;
+        float Ta = h->threshold;
+        const void* fa = nullptr;
+        uint8_t* c = codes;
+        void* args[] = {&ga, &Ta, &fa, &c};
+        if (hipModuleLaunchKernel(h->k_codes, blocks, 1, 1, 256, 1, 1, 0, h->stream, args, nullptr) != hipSuccess)
+            return fail(h, HPMC_ERR_HIP, "codes kernel launch failed");
+    }
+    if (hipGetLastError() != hipSuccess) return fail(h, HPMC_ERR_HIP, "codes kernel launch failed");
+    ok(h);
+    return HPMC_OK;
+}
+
+uint64_t HPMCgetLevel4(struct HPMCHistoPyramid* h, uint32_t* out, uint64_t capacity)
+{
+    if (!h || !out) {
+        fail(h, HPMC_ERR_NULL, "NULL argument");
+        return 0;
+    }
+    if (h->broken || h->tainted || !h->built) {
+        fail(h, HPMC_ERR_ARG, "HistoPyramid not built");
+        return 0;
+    }
+    if (capacity < h->n_level4) {
+        fail(h, HPMC_ERR_CAPACITY, "need %llu entries", (unsigned long long)h->n_level4);
+        return 0;
+    }
+    if (hipMemcpyAsync(out, h->level4, h->n_level4 * sizeof(uint32_t), hipMemcpyDeviceToDevice, h->stream) !=
+        hipSuccess) {
+        fail(h, HPMC_ERR_HIP, "copy failed");
+        return 0;
+    }
+    ok(h);
+    return h->n_level4;
+}
+
+int32_t HPMCsetProfiling(struct HPMCHistoPyramid* h, int32_t enable)
+{
+    if (!h) return fail(nullptr, HPMC_ERR_NULL, "h == NULL");
+    h->profiling = enable != 0;
+    ok(h);
+    return HPMC_OK;
+}
+
+int32_t HPMCgetStageTimes(struct HPMCHistoPyramid* h, double* ms, uint32_t* launches)
+{
+    if (!h || !ms) return fail(h, HPMC_ERR_NULL, "NULL argument");
+    for (int st = 0; st < 3; ++st) {
+        std::vector<hipEvent_t>& ev = h->stage_events[st];
+        double total = 0.0;
+        uint32_t n = 0;
+        for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+            if (hipEventSynchronize(ev[i + 1]) != hipSuccess) return fail(h, HPMC_ERR_HIP, "event sync failed");
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) == hipSuccess) {
+                total += t;
+                ++n;
+            }
+        }
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        ms[st] = total;
+        if (launches) launches[st] = n;
+    }
+    ok(h);
+    return HPMC_OK;
+}
+
+int32_t HPMCcheckFieldSnippet(const char* source, char* log, size_t log_bytes)
+{
+    if (!source) return fail(nullptr, HPMC_ERR_NULL, "source == NULL");
+    std::string text;
+    std::vector<char> code = rtc_compile(source, &text);
+    if (log && log_bytes) {
+        size_t n = text.size() < log_bytes - 1 ? text.size() : log_bytes - 1;
+        memcpy(log, text.data(), n);
+        log[n] = '\0';
+    }
+    if (code.empty()) return fail(nullptr, HPMC_ERR_COMPILE, "field snippet failed to compile: %s", text.c_str());
+    ok(nullptr);
+    return HPMC_OK;
+}
+
+int32_t HPMCgetLayout(struct HPMCHistoPyramid* h, int32_t out[5])
+{
+    if (!h || !out) return fail(h, HPMC_ERR_NULL, "NULL argument");
+    if (!determine_layout(h)) return h->last_error;
+    out[0] = h->tile[0];
+    out[1] = h->tile[1];
+    out[2] = h->tiles[0];
+    out[3] = h->tiles[1];
+    out[4] = h->l2;
+    ok(h);
+    return HPMC_OK;
+}
+
+}  // extern "C"

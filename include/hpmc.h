@@ -1,0 +1,211 @@
+/*
+ * hpmc.h -- C ABI of the MI355X HistoPyramid marching-cubes engine.
+ *
+ * A drop-in for the C API of HPMC 1.0.1 (hpmc/include/hpmc.h:295-544): the
+ * same function names, arity and call order, with OpenGL object names
+ * replaced by HIP device memory.  No OpenGL header is needed.
+ *
+ *   GL type     -> here
+ *   GLsizei     -> int32_t        GLfloat -> float
+ *   GLuint      -> uint32_t       GLboolean -> uint8_t
+ *   3D texture  -> device pointer to an FP32 lattice, x fastest
+ *   GLSL fetch snippet -> HIP C++ snippet compiled with hipRTC
+ *   display program / transform feedback -> device output buffer (N3F_V3F)
+ *
+ * Errors follow the reference: no call throws; failures return 0/false/NULL
+ * and latch the HistoPyramid as broken until it is reconfigured by a setter
+ * (hpmc/src/hpiface.cpp:100-200).  HPMCgetLastError adds a reason.
+ * All device work of a HistoPyramid runs in order on its HIP stream (the
+ * null stream unless HPMCsetStream names another); HPMCacquireNumberOfVertices
+ * is the one call that waits for the device (hpmc.h:211-215 of the reference).
+ */
+#ifndef HPMC_AMD_HPMC_H
+#define HPMC_AMD_HPMC_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct HPMCConstants;
+struct HPMCHistoPyramid;
+struct HPMCTraversalHandle;
+
+/* ---- constants --------------------------------------------------------- */
+
+/* Replaces HPMCcreateConstants (hpmc.h:310-311, constants.cpp:54-347).
+ * Uploads the marching-cubes tables to the current HIP device.  The GL
+ * version caps are accepted and ignored. */
+struct HPMCConstants* HPMCcreateConstants(int32_t max_gl_major, int32_t max_gl_minor);
+
+/* Replaces HPMCdestroyConstants (hpmc.h:319-320, constants.cpp:350-392). */
+void HPMCdestroyConstants(struct HPMCConstants* c);
+
+/* ---- HistoPyramid configuration ---------------------------------------- */
+
+/* Replaces HPMCcreateHistoPyramid (hpmc.h:330-331, hpiface.cpp:45-97).
+ * The new HistoPyramid is tainted and broken until configured. */
+struct HPMCHistoPyramid* HPMCcreateHistoPyramid(struct HPMCConstants* c);
+
+/* Replaces HPMCsetLatticeSize (hpmc.h:346-350, hpiface.cpp:100-114).
+ * Also resets the grid size to lattice - 1. */
+void HPMCsetLatticeSize(struct HPMCHistoPyramid* h, int32_t x_size, int32_t y_size, int32_t z_size);
+
+/* Replace HPMCsetFieldAsBinary / HPMCsetFieldAsContinuous (hpmc.h:352-356,
+ * hpiface.cpp:131-142).  Binary fields classify against 0.5. */
+void HPMCsetFieldAsBinary(struct HPMCHistoPyramid* h);
+void HPMCsetFieldAsContinuous(struct HPMCHistoPyramid* h);
+
+/* Replaces HPMCsetGridSize (hpmc.h:375-379, hpiface.cpp:117-128). */
+void HPMCsetGridSize(struct HPMCHistoPyramid* h, int32_t x_size, int32_t y_size, int32_t z_size);
+
+/* Replaces HPMCsetGridExtent (hpmc.h:392-396, hpiface.cpp:145-161). */
+void HPMCsetGridExtent(struct HPMCHistoPyramid* h, float x_extent, float y_extent, float z_extent);
+
+/* Replaces HPMCsetFieldTexture3D (hpmc.h:405-408, hpiface.cpp:164-184).
+ * `volume` is a device pointer to lattice_x*lattice_y*lattice_z FP32
+ * samples, x fastest.  Samples beyond the lattice (forward differences at
+ * the upper faces) clamp to the edge sample, as a NEAREST/CLAMP texture. */
+void HPMCsetFieldTexture3D(struct HPMCHistoPyramid* h, const float* volume, uint8_t gradient);
+
+/* Replaces HPMCsetFieldCustom (hpmc.h:439-443, hpiface.cpp:187-200).
+ * `source` is HIP C++ defining
+ *     __device__ float HPMC_fetch(float3 p);
+ * evaluated at texel centres p = ((i+0.5)/X, (j+0.5)/Y, (k+0.5)/Z), exactly
+ * as the GLSL contract (hpmc.h:409-437).  Values the snippet reads at run
+ * time are declared as __device__ globals and set with HPMCsetFieldUniform.
+ * `builder_texunit` is accepted and ignored. */
+void HPMCsetFieldCustom(struct HPMCHistoPyramid* h, const char* source, uint32_t builder_texunit,
+                        uint8_t gradient);
+
+/* Replaces HPMCgetBuilderProgram (hpmc.h:445-446, hpiface.cpp:203-222).
+ * Sets the HistoPyramid up if needed and returns a non-zero id of the
+ * compiled builder, or 0 when broken or when setup fails. */
+uint32_t HPMCgetBuilderProgram(struct HPMCHistoPyramid* h);
+
+/* Declared at hpmc.h:450-452 but never defined by the reference: frees the
+ * HistoPyramid and its device memory. */
+void HPMCdestroyHandle(struct HPMCHistoPyramid* h);
+
+/* ---- build and count ---------------------------------------------------- */
+
+/* Replaces HPMCbuildHistopyramid (hpmc.h:460-462, hpiface.cpp:225-290).
+ * Enqueues classification and the pyramid reduction; does not wait. */
+void HPMCbuildHistopyramid(struct HPMCHistoPyramid* h, float threshold);
+
+/* Replaces HPMCacquireNumberOfVertices (hpmc.h:468-469, hpiface.cpp:293-345).
+ * Waits for the last build and returns its vertex count. */
+uint32_t HPMCacquireNumberOfVertices(struct HPMCHistoPyramid* h);
+
+/* ---- traversal and extraction ------------------------------------------ */
+
+/* Replaces HPMCcreateTraversalHandle (hpmc.h:477-478, triface.cpp:43-109). */
+struct HPMCTraversalHandle* HPMCcreateTraversalHandle(struct HPMCHistoPyramid* h);
+
+/* Replaces HPMCdestroyTraversalHandle (hpmc.h:484-485, triface.cpp:112-122). */
+void HPMCdestroyTraversalHandle(struct HPMCTraversalHandle* th);
+
+/* Replaces HPMCgetTraversalShaderFunctions (hpmc.h:493-494,
+ * triface.cpp:125-192): returns HIP device source (layout constants plus
+ * the device functions) for callers that fuse extraction into their own
+ * hipRTC kernels.  The caller frees it with free(). */
+char* HPMCgetTraversalShaderFunctions(struct HPMCTraversalHandle* th);
+
+/* Replaces HPMCsetTraversalHandleProgram (hpmc.h:517-522,
+ * triface.cpp:195-327).  There is no GL program: the call validates its
+ * arguments like the reference (non-zero program, distinct units) and
+ * records the program id; output goes to the buffer bound with
+ * HPMCsetTraversalHandleBuffer. */
+bool HPMCsetTraversalHandleProgram(struct HPMCTraversalHandle* th, uint32_t program,
+                                   uint32_t tex_unit_work1, uint32_t tex_unit_work2,
+                                   uint32_t tex_unit_work3);
+
+/* Replace HPMCextractVertices and its transform-feedback variants
+ * (hpmc.h:535-545, triface.cpp:332-500).  All four write the same thing:
+ * per vertex the normal then the position, 6 floats (N3F_V3F, as the
+ * reference's transform-feedback demo captures them,
+ * apps/transform_feedback/transform_feedback.cpp:292-296), in the
+ * reference's extraction order, into the bound buffer.  Enqueued; does not
+ * wait.  Vertices beyond the buffer's capacity are dropped and counted. */
+bool HPMCextractVertices(struct HPMCTraversalHandle* th);
+bool HPMCextractVerticesTransformFeedback(struct HPMCTraversalHandle* th);
+bool HPMCextractVerticesTransformFeedbackNV(struct HPMCTraversalHandle* th);
+bool HPMCextractVerticesTransformFeedbackEXT(struct HPMCTraversalHandle* th);
+
+/* ---- additions (no counterpart in the reference) ----------------------- */
+
+/* Error codes returned by the additions and by HPMCgetLastError. */
+enum {
+    HPMC_OK = 0,
+    HPMC_ERR_NULL = 1,          /* NULL handle or pointer */
+    HPMC_ERR_BROKEN = 2,        /* handle latched broken */
+    HPMC_ERR_SIZE = 3,          /* lattice / grid size rejected */
+    HPMC_ERR_FIELD = 4,         /* no field source */
+    HPMC_ERR_COMPILE = 5,       /* hipRTC failed on the snippet */
+    HPMC_ERR_HIP = 6,           /* HIP runtime error */
+    HPMC_ERR_CAPACITY = 7,      /* output buffer too small */
+    HPMC_ERR_ARG = 8,           /* invalid argument */
+    HPMC_ERR_UNSUPPORTED = 9    /* configuration not implemented */
+};
+
+/* Last error of a HistoPyramid (or of the calling thread when h is NULL)
+ * and its message; the message stays valid until the next call on h. */
+int32_t HPMCgetLastError(struct HPMCHistoPyramid* h);
+const char* HPMCgetLastErrorString(struct HPMCHistoPyramid* h);
+
+/* HIP stream (hipStream_t) for all device work of h; NULL = null stream. */
+int32_t HPMCsetStream(struct HPMCHistoPyramid* h, void* hip_stream);
+
+/* Sets a __device__ global of the field snippet, by name, from host memory
+ * (takes the place of glUniform* on HPMCgetBuilderProgram's program,
+ * apps/metaballs/metaballs.cpp:277-291).  Stream-ordered with builds. */
+int32_t HPMCsetFieldUniform(struct HPMCHistoPyramid* h, const char* name, const void* data, size_t bytes);
+
+/* Binds the extraction output: a device buffer of capacity_vertices * 6
+ * floats (N3F_V3F).  Replaces the transform-feedback buffer. */
+int32_t HPMCsetTraversalHandleBuffer(struct HPMCTraversalHandle* th, float* device_n3f_v3f,
+                                     uint64_t capacity_vertices);
+
+/* Optionally binds a second output: per vertex {uint32 cell, uint32 edge},
+ * cell = x + cells_x*(y + cells_y*z), edge = 4*axis + s0 + 2*s1 with (s0,s1)
+ * the edge's start offsets along the other two axes (x,y,z order).
+ * NULL unbinds. */
+int32_t HPMCsetTraversalHandleCellEdgeBuffer(struct HPMCTraversalHandle* th, void* device_records,
+                                             uint64_t capacity_vertices);
+
+/* Vertices the last extraction dropped for lack of capacity (waits). */
+uint32_t HPMCacquireDroppedVertices(struct HPMCTraversalHandle* th);
+
+/* Writes the HPMC corner code of every cell (x fastest over the grid) for
+ * the last build's threshold into a device buffer of cells_x*cells_y*cells_z
+ * bytes.  Enqueued. */
+int32_t HPMCgetCellCodes(struct HPMCHistoPyramid* h, uint8_t* device_codes);
+
+/* Copies the pyramid level holding 16x16-cell sums (level 4 of the
+ * reference's base-level pyramid), indexed by Morton block code, into a
+ * device buffer; returns the entry count, or 0 on failure. */
+uint64_t HPMCgetLevel4(struct HPMCHistoPyramid* h, uint32_t* device_out, uint64_t capacity);
+
+/* Per-stage timing with HIP events on the HistoPyramid's stream: when
+ * enabled, every launch of classify, scan and emit is bracketed by a pair of
+ * events.  HPMCgetStageTimes waits for them, writes the summed milliseconds
+ * (ms[3] = classify, scan, emit) and launch counts, and starts over. */
+int32_t HPMCsetProfiling(struct HPMCHistoPyramid* h, int32_t enable);
+int32_t HPMCgetStageTimes(struct HPMCHistoPyramid* h, double* ms, uint32_t* launches);
+
+/* Compiles a field snippet (as HPMCsetFieldCustom takes it) for gfx950
+ * without touching a device; the compiler log goes to `log`.  Returns
+ * HPMC_OK or HPMC_ERR_COMPILE. */
+int32_t HPMCcheckFieldSnippet(const char* source, char* log, size_t log_bytes);
+
+/* Layout of the reference base level (hpmc/src/init.cpp:105-130):
+ * out = {tile_x, tile_y, tiles_x, tiles_y, size_l2}. */
+int32_t HPMCgetLayout(struct HPMCHistoPyramid* h, int32_t out[5]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,89 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+include/hpmc.h declares, compiles field snippets with hipRTC, and follows
+the reference's NULL-handle conventions (hpmc/src/hpiface.cpp:48-53,
+hpmc/src/triface.cpp:46-51, 339-349) without touching a device."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import hpmc_amd
+from hpmc_amd import workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "hpmc.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(HPMC[A-Za-z0-9]+)\s*\(", text)))
+
+
+def test_header_declares_reference_api():
+    names = set(_declared())
+    reference_api = {
+        "HPMCcreateConstants", "HPMCdestroyConstants", "HPMCcreateHistoPyramid", "HPMCsetLatticeSize",
+        "HPMCsetFieldAsBinary", "HPMCsetFieldAsContinuous", "HPMCsetGridSize", "HPMCsetGridExtent",
+        "HPMCsetFieldTexture3D", "HPMCsetFieldCustom", "HPMCgetBuilderProgram", "HPMCdestroyHandle",
+        "HPMCbuildHistopyramid", "HPMCacquireNumberOfVertices", "HPMCcreateTraversalHandle",
+        "HPMCdestroyTraversalHandle", "HPMCgetTraversalShaderFunctions", "HPMCsetTraversalHandleProgram",
+        "HPMCextractVertices", "HPMCextractVerticesTransformFeedback", "HPMCextractVerticesTransformFeedbackNV",
+        "HPMCextractVerticesTransformFeedbackEXT",
+    }
+    assert reference_api <= names
+
+
+def test_library_exports_every_declared_symbol():
+    L = hpmc_amd.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", hpmc_amd.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\bT (HPMC\w+)", out))
+    for name in _declared():
+        assert name in exported, name
+        assert getattr(L, name) is not None
+    bound = {n for n, _, _ in hpmc_amd.SIGNATURES}
+    assert bound == set(_declared())
+
+
+def test_null_handles_follow_reference_conventions():
+    L = hpmc_amd.lib()
+    assert not L.HPMCcreateHistoPyramid(None)
+    assert L.HPMCgetLastError(None) == 1
+    L.HPMCsetLatticeSize(None, 8, 8, 8)
+    L.HPMCsetGridSize(None, 7, 7, 7)
+    L.HPMCsetGridExtent(None, 1.0, 1.0, 1.0)
+    L.HPMCsetFieldTexture3D(None, None, 0)
+    L.HPMCsetFieldCustom(None, b"", 0, 0)
+    L.HPMCbuildHistopyramid(None, 0.5)
+    assert L.HPMCgetBuilderProgram(None) == 0
+    assert L.HPMCacquireNumberOfVertices(None) == 0
+    assert not L.HPMCcreateTraversalHandle(None)
+    assert not L.HPMCgetTraversalShaderFunctions(None)
+    assert not L.HPMCsetTraversalHandleProgram(None, 1, 0, 1, 2)
+    for fn in (L.HPMCextractVertices, L.HPMCextractVerticesTransformFeedback,
+               L.HPMCextractVerticesTransformFeedbackNV, L.HPMCextractVerticesTransformFeedbackEXT):
+        assert not fn(None)
+    L.HPMCdestroyHandle(None)
+    L.HPMCdestroyTraversalHandle(None)
+    L.HPMCdestroyConstants(None)
+
+
+@pytest.mark.parametrize("src", [workloads.CAYLEY_SNIPPET, workloads.METABALL_SNIPPET])
+def test_snippets_compile_for_gfx950(src):
+    hpmc_amd.check_snippet(src)
+
+
+def test_bad_snippet_reports_compile_error():
+    with pytest.raises(hpmc_amd.HPMCError) as e:
+        hpmc_amd.check_snippet("__device__ float HPMC_fetch(float3 p) { return undefined_name; }")
+    assert "HPMC_ERR_COMPILE" in str(e.value) and "undefined_name" in str(e.value)
+    with pytest.raises(hpmc_amd.HPMCError):
+        hpmc_amd.check_snippet("__device__ float not_the_fetch(float3 p) { return 0.0f; }")
+
+
+def test_traversal_source_is_valid_hip():
+    # the device source HPMCgetTraversalShaderFunctions hands out compiles
+    src = open(os.path.join(ROOT, "hpmc_amd", "csrc", "hpmc_device.h")).read()
+    assert "emit_body" in src and "classify_body" in src

@@ -1,0 +1,198 @@
+"""Pins the CPU oracle (oracle/) independently of its own formulas.
+
+layout      known answers (64^3, 256^3, 512^3) and the closed form of
+            hpmc/src/init.cpp:105-130 recomputed in FP64 for power-of-two
+            friendly sizes
+classify    pure-Python per-cell brute force, constant and plane fields
+traverse    literal HistoPyramid walk == Morton closed form, key round trip,
+            hand-worked order on a 2-slice example
+emit        linear fields give exact t and positions, a sphere's distance
+            function gives |p - c| ~ r
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from hpmc_amd import workloads
+
+
+# ---------------------------------------------------------------- layout ---
+@pytest.mark.parametrize("n,expect", [
+    (64, (32, 32, 8, 8, 8)),
+    (256, (128, 128, 16, 16, 11)),
+    (512, (256, 256, 32, 32, 13)),
+])
+def test_layout_known_answers(n, expect):
+    assert oracle.layout((n, n, n), (n - 1, n - 1, n - 1)) == expect
+
+
+def test_layout_closed_form():
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        cells = [int(v) for v in rng.integers(2, 700, size=3)]
+        size = [c + int(rng.integers(1, 3)) for c in cells]
+        lay = oracle.layout(size, cells)
+        ts = [1 << int(np.ceil(np.log2(c / 2.0))) for c in cells[:2]]
+        tiles_x = 1 << max(0, int(np.ceil(np.log2(np.sqrt(cells[2] / (ts[0] / ts[1]))))))
+        tiles_y = -(-cells[2] // tiles_x)
+        S = 1 << int(np.ceil(np.log2(max(ts[0] * tiles_x, ts[1] * tiles_y))))
+        assert lay == (ts[0], ts[1], S // ts[0], S // ts[1], int(np.log2(S))), (cells, lay)
+        # every slice fits the base image
+        assert lay[2] * lay[3] >= cells[2]
+
+
+def test_layout_rejects_bad_sizes():
+    assert oracle.layout((1, 8, 8), (0, 7, 7)) is None
+    assert oracle.layout((16385, 8, 8), (16384, 7, 7)) is None
+    assert oracle.layout((8, 8, 8), (8, 7, 7)) is None
+
+
+# -------------------------------------------------------------- classify ---
+def _brute_codes(f, cells, T):
+    cz, cy, cx = cells[2], cells[1], cells[0]
+    codes = np.zeros((cz, cy, cx), np.uint8)
+    for z, y, x in itertools.product(range(cz), range(cy), range(cx)):
+        c = 0
+        for dz, dy, dx in itertools.product((0, 1), (0, 1), (0, 1)):
+            if f[z + dz, y + dy, x + dx] < T:
+                c |= 1 << (dx + 2 * dy + 4 * dz)
+        codes[z, y, x] = c
+    return codes
+
+
+@pytest.mark.parametrize("shape,cells", [
+    ((5, 6, 7), None), ((9, 4, 3), None), ((8, 8, 8), (5, 6, 4)), ((3, 3, 17), None),
+])
+def test_classify_brute_force(shape, cells):
+    rng = np.random.default_rng(sum(shape))
+    f = rng.random((shape[2], shape[1], shape[0])).astype(np.float32)
+    cells = cells or (shape[0] - 1, shape[1] - 1, shape[2] - 1)
+    codes, counts = oracle.classify(f, 0.5, cells=cells)
+    assert np.array_equal(codes, _brute_codes(f, cells, 0.5))
+    assert np.array_equal(counts, oracle.CASE_COUNTS[codes])
+
+
+def test_classify_constant_and_plane():
+    f = np.full((6, 6, 6), 2.0, np.float32)
+    codes, counts = oracle.classify(f, 1.0)
+    assert codes.max() == 0 and counts.sum() == 0
+    codes, counts = oracle.classify(f, 3.0)
+    assert (codes == 255).all() and counts.sum() == 0
+    # plane f = x - 2.5: corners with x <= 2 are inside (bits x=0), cells at
+    # x = 2 are crossed: code = bits where corner x == 0 -> 0b01010101
+    x = np.arange(7, dtype=np.float32)
+    f = np.broadcast_to(x[None, None, :] - 2.5, (5, 6, 7)).copy()
+    codes, counts = oracle.classify(f, 0.0)
+    assert (codes[:, :, 2] == 0x55).all()
+    assert (codes[:, :, :2] == 255).all() and (codes[:, :, 3:] == 0).all()
+    assert (counts[:, :, 2] == 6).all()   # a quad cut: two triangles
+
+
+# -------------------------------------------------------------- traverse ---
+@pytest.mark.parametrize("shape", [(12, 9, 7), (17, 17, 17), (10, 20, 6), (33, 5, 9), (9, 9, 40)])
+def test_hp_literal_equals_morton_closed_form(shape):
+    rng = np.random.default_rng(shape[0] * 7 + shape[2])
+    f = rng.random((shape[2], shape[1], shape[0])).astype(np.float32)
+    cells = (shape[0] - 1, shape[1] - 1, shape[2] - 1)
+    codes, counts = oracle.classify(f, 0.5)
+    lay = oracle.layout(shape, cells)
+    c1, s1 = oracle.vertex_list(counts, cells, lay, literal=True)
+    c2, s2 = oracle.vertex_list(counts, cells, lay, literal=False)
+    assert len(c1) == counts.sum() and np.array_equal(c1, c2) and np.array_equal(s1, s2)
+    # key round trip: each (cell, slot) appears once and slots run 0..count-1
+    pairs = list(zip(c1.tolist(), s1.tolist()))
+    assert len(set(pairs)) == len(pairs)
+    flat = counts.reshape(-1)
+    for c, s in pairs:
+        assert s < flat[c]
+
+
+def test_traversal_order_hand_worked():
+    # 3x3x3 lattice: cells 2x2x2, tile 1x1 texels (2x2 cells), 2 slices laid
+    # out side by side along x (tiles_x = 2).  Doubled image positions:
+    # slice 0 cells at X in {0,1}, slice 1 at X in {2,3}; Morton (x low)
+    # order is (0,0),(1,0),(0,1),(1,1) of slice 0, then the same of slice 1.
+    cells = (2, 2, 2)
+    lay = oracle.layout((3, 3, 3), cells)
+    assert lay == (1, 1, 2, 2, 1)
+    counts = np.full((2, 2, 2), 3, np.uint8)
+    c, s = oracle.vertex_list(counts, cells, lay, literal=True)
+    order = [c[i] for i in range(0, len(c), 3)]
+    assert order == [0, 1, 2, 3, 4, 5, 6, 7]
+    assert s.tolist() == [0, 1, 2] * 8
+
+
+def test_level4_sums():
+    f = workloads.cayley((40, 37, 35))
+    cells = (39, 36, 34)
+    codes, counts = oracle.classify(f, 0.0)
+    lay = oracle.layout((40, 37, 35), cells)
+    l4 = oracle.level4(counts, cells, lay)
+    assert l4.sum() == counts.sum()
+    cell, _ = oracle.vertex_list(counts, cells, lay)
+    # level-4 blocks appear in the list as contiguous runs in Morton order
+    nz = np.nonzero(l4)[0]
+    runs = np.cumsum(l4[nz])
+    assert runs[-1] == len(cell)
+
+
+# ------------------------------------------------------------------ emit ---
+def _list(f, T, ext=(1.0, 1.0, 1.0)):
+    shape = f.shape[::-1]
+    cells = tuple(v - 1 for v in shape)
+    codes, counts = oracle.classify(f, T)
+    lay = oracle.layout(shape, cells)
+    cell, slot = oracle.vertex_list(counts, cells, lay)
+    return codes, cell, slot, cells
+
+
+def test_emit_linear_field_exact():
+    # f = x + 0.25 y (lattice units), T = 3.3: crossings on x-edges at
+    # x = 3.3 - 0.25 y exactly; positions p = ext * (k + t*axis) / cells
+    X, Y, Z = 9, 5, 4
+    x = np.arange(X, dtype=np.float64)
+    y = np.arange(Y, dtype=np.float64)
+    f = (x[None, None, :] + 0.25 * y[None, :, None] + 0 * np.zeros((Z, 1, 1))).astype(np.float32)
+    codes, cell, slot, cells = _list(f, 3.3, ext=(2.0, 1.0, 1.0))
+    out, t = oracle.emit(f, codes, cell, slot, 3.3, ext=(2.0, 1.0, 1.0))
+    assert len(out) > 0
+    assert ((t >= 0) & (t <= 1)).all()
+    px = out[:, 3] * cells[0] / 2.0
+    py = out[:, 4] * cells[1]
+    vals = px + 0.25 * py
+    np.testing.assert_allclose(vals, 3.3, atol=1e-6)
+
+
+def test_emit_sphere_distance():
+    n = 48
+    p = (np.arange(n) + 0.5) / n
+    d = np.sqrt((p[None, None, :] - 0.5) ** 2 + (p[None, :, None] - 0.5) ** 2 + (p[:, None, None] - 0.5) ** 2)
+    f = (d - 0.3).astype(np.float32)
+    codes, cell, slot, cells = _list(f, 0.0)
+    out, t = oracle.emit(f, codes, cell, slot, 0.0)
+    # positions are in [0, ext] grid coordinates: lattice point k sits at
+    # k / cells, the sphere centre (texel coordinate 0.5) at (0.5 n - 0.5) / cells
+    c = (0.5 * n - 0.5) / cells[0]
+    r = np.linalg.norm(out[:, 3:] - c, axis=1) * cells[0] / n
+    np.testing.assert_allclose(r, 0.3, atol=2e-3)
+    # forward-difference normals point along the outward radius, as
+    # T - f(p + e) = -(grad f) h
+    radial = (out[:, 3:] - c) / np.linalg.norm(out[:, 3:] - c, axis=1)[:, None]
+    nn = out[:, :3] / np.linalg.norm(out[:, :3], axis=1)[:, None]
+    assert (np.einsum("ij,ij->i", nn, radial) < -0.9).all()
+
+
+def test_emit_f32_matches_f64():
+    f = workloads.cayley((33, 31, 29))
+    codes, cell, slot, cells = _list(f, 0.0)
+    o64, _ = oracle.emit(f, codes, cell, slot, 0.0)
+    o32 = oracle.emit(f, codes, cell, slot, 0.0, fp64=False)
+    np.testing.assert_allclose(o32, o64, rtol=1e-5, atol=1e-6)
+
+
+def test_run_volume_counts():
+    f = workloads.cayley((40, 40, 40))
+    codes, counts = oracle.classify(f, 0.0)
+    assert oracle.run_volume(f, 0.0) == counts.sum()
