@@ -107,29 +107,105 @@ struct VolumeField {
         k = min(max(k, 0), sz - 1);
         return v[((size_t)k * sy + j) * sx + i];
     }
-    // In-range load without clamping (classification rows).
-    __device__ __forceinline__ float row(int i, int j, int k) const
+    // In-range load without clamping (classification rows): a row pointer
+    // that is uniform across the wave plus the lane's x offset.
+    __device__ __forceinline__ float row(unsigned i, int j, int k) const
     {
-        return v[((size_t)k * sy + j) * sx + i];
+        const char* rp = (const char*)(v + ((size_t)k * sy + j) * sx);
+        return *(const float*)(rp + i * 4u);
     }
 };
 
 // ---------------------------------------------------------------------------
 // classify: one wave per (64 x 16 cells) x zseg-slice column.
 //
-// Each plane is read once as 17 rows of 64 samples (+1 column), turned into
-// 64-bit ballot masks of (f < T), and the codes of a whole cell row come out
-// of four masks.  Rows whose masks are uniform cannot hold a surface cell and
-// are skipped with scalar tests only.  Counts are summed per lane over the 16
-// rows of a block and across 16-lane groups, giving four level-4 sums per
-// slice.
+// Each plane is read once as 17 rows of 64 samples plus the 17 samples of
+// the next column (one vector load), turned into 64-bit ballot masks of
+// (f < T), and the codes of a whole cell row come out of four masks.  Loads
+// use clamped, always-valid addresses so a plane's 18 loads issue back to
+// back, and the next plane is loaded while the current one is classified.
+// Slices whose two planes are uniformly outside or inside cannot hold a
+// surface cell and are skipped with scalar tests.  Counts are summed per
+// lane over the 16 rows of a block and across 16-lane groups, giving four
+// level-4 sums per slice.
+template <class F>
+__device__ __forceinline__ void classify_load_plane(const F& field, unsigned xl, int y0, int ymax, unsigned xe,
+                                                    int ye, int zz, float (&v)[kBlock + 1], float& ve)
+{
+#pragma unroll
+    for (int r = 0; r <= kBlock; ++r) v[r] = field.row(xl, min(y0 + r, ymax), zz);
+    ve = field.row(xe, ye, zz);
+}
+
+// Per-wave state carried from plane to plane.
+struct ClassifyCarry {
+    uint64_t win;   // per lane: 2-bit windows of the previous plane's 17 rows
+    uint64_t any;   // previous plane: some sample inside
+    uint64_t all;   // previous plane: every sample inside
+};
+
+struct ClassifyWave {
+    int lane, x0, y0, x, sy, cx, cy;
+    bool xin, xcell, ein, lane_e;
+    uint64_t lane63;
+};
+
+// Classifies plane `zz` against the carried plane zz-1 (when have_lo) and
+// writes the four level-4 sums of slice zz-1.
+__device__ __forceinline__ void classify_plane(const ClassifyWave& w, const Geom& g, float T,
+                                               const float (&v)[kBlock + 1], float ve, bool have_lo, int zz,
+                                               ClassifyCarry& c, uint32_t* __restrict__ level4,
+                                               const uint8_t* __restrict__ lut)
+{
+    // bit r of eb: the sample at (x0+64, y0+r) is inside
+    const uint64_t eb = __ballot(w.lane_e && ve < T);
+    uint64_t win = 0, any = eb, all = ~0ull;
+#pragma unroll
+    for (int r = 0; r <= kBlock; ++r) {
+        const uint64_t m = __ballot(w.xin && (w.y0 + r < w.sy) && v[r] < T);
+        any |= m;
+        all &= m;
+        const uint64_t e = (eb >> r) & 1ull;
+        const uint64_t bits = ((m >> w.lane) & 3ull) | ((e & w.lane63) << 1);
+        win |= bits << (2 * r);
+    }
+    // all-inside needs every row's +1 column inside as well
+    const uint64_t erows = (1ull << (kBlock + 1)) - 1ull;
+    if (!w.ein || (eb & erows) != erows) all = 0;
+    if (have_lo) {
+        const int z = zz - 1;
+        uint32_t acc = 0;
+        const bool uniform_out = (c.any | any) == 0ull;
+        const bool uniform_in = (c.all & all) == ~0ull;
+        if (!(uniform_out || uniform_in)) {
+#pragma unroll
+            for (int r = 0; r < kBlock; ++r) {
+                const uint32_t code = (uint32_t)((c.win >> (2 * r)) & 0xf) | ((uint32_t)((win >> (2 * r)) & 0xf) << 4);
+                const bool valid = w.xcell && (w.y0 + r < w.cy);
+                acc += valid ? lut[code] : 0u;
+            }
+        }
+        // 16-lane group sums = level-4 sums of four blocks.
+        acc += __shfl_xor(acc, 8, 16);
+        acc += __shfl_xor(acc, 4, 16);
+        acc += __shfl_xor(acc, 2, 16);
+        acc += __shfl_xor(acc, 1, 16);
+        if ((w.lane & 15) == 0 && w.x < w.cx) level4[block_key(g, w.x, w.y0, z)] = acc;
+    }
+    c.win = win;
+    c.any = any;
+    c.all = all;
+}
+
 template <class F>
 __device__ __forceinline__ void classify_body(const F& field, const Geom& g, float T,
                                               uint32_t* __restrict__ level4, int zseg,
                                               const uint8_t* __restrict__ lut)
 {
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // wave-uniform: keeps the column indices, plane loop and row pointers in
+    // scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     const int npx = (g.cx + 63) >> 6;
     const int npy = (g.cy + kBlock - 1) >> kBlockLog2;
     const int npz = (g.cz + zseg - 1) / zseg;
@@ -138,64 +214,42 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
     const int rest = wave / npx;
     const int py = rest % npy;
     const int pz = rest / npy;
-    const int x0 = px << 6, y0 = py << kBlockLog2;
+    ClassifyWave w;
+    w.lane = lane;
+    w.x0 = px << 6;
+    w.y0 = py << kBlockLog2;
+    w.x = w.x0 + lane;
+    w.sy = g.sy;
+    w.cx = g.cx;
+    w.cy = g.cy;
+    w.xin = w.x < g.sx;
+    w.xcell = w.x < g.cx;
+    w.ein = w.x0 + 64 < g.sx;
+    w.lane_e = w.ein && lane <= kBlock && (w.y0 + lane < g.sy);
+    w.lane63 = lane == 63 ? 1ull : 0ull;
     const int z0 = pz * zseg;
-    const int z1 = min(z0 + zseg, g.cz);   // cell slices [z0, z1)
-    const int x = x0 + lane;
-    const bool xin = x < g.sx;
-    const bool xcell = x < g.cx;
-    const bool ein = x0 + 64 < g.sx;
+    const int z1 = min(z0 + zseg, g.cz);   // cell slices [z0, z1): planes z0..z1
+    const unsigned xl = (unsigned)min(w.x, g.sx - 1);
+    const unsigned xe = (unsigned)min(w.x0 + 64, g.sx - 1);
+    const int ye = min(w.y0 + min(lane, kBlock), g.sy - 1);
+    const int ymax = g.sy - 1;
 
-    // Per lane: 2-bit window (bit at x, bit at x+1) of each of the 17 rows,
-    // packed as bits 2r..2r+1 of a 64-bit word; plus plane-wide scalar masks.
-    uint64_t win_lo = 0, any_lo = 0, all_lo = ~0ull;
-    bool alle_lo = true;
-    for (int zz = z0; zz <= z1; ++zz) {
-        uint64_t win = 0, any = 0, all = ~0ull;
-        bool alle = true;
-#pragma unroll
-        for (int r = 0; r <= kBlock; ++r) {
-            const int y = y0 + r;
-            const bool yin = y < g.sy;
-            float v = (xin && yin) ? field.row(x, y, zz) : 0.0f;
-            const uint64_t m = __ballot((xin && yin) && v < T);
-            float ve = (ein && yin) ? field.row(x0 + 64, y, zz) : 0.0f;
-            const uint64_t e = (ein && yin && ve < T) ? 1ull : 0ull;
-            any |= m | e;
-            all &= m;
-            alle = alle && (e != 0ull);
-            uint64_t nb = lane == 63 ? e : (m >> (lane + 1)) & 1ull;
-            win |= (((m >> lane) & 1ull) | (nb << 1)) << (2 * r);
-        }
-        if (zz > z0) {
-            const int z = zz - 1;
-            uint32_t acc = 0;
-            // A cell row is empty when its four rows are all outside or all
-            // inside; test the two planes at once and skip the slice.
-            const bool uniform_out = (any_lo | any) == 0ull;
-            const bool uniform_in = (all_lo & all) == ~0ull && ein && alle_lo && alle;
-            if (!(uniform_out || uniform_in)) {
-#pragma unroll
-                for (int r = 0; r < kBlock; ++r) {
-                    const uint32_t code = (uint32_t)((win_lo >> (2 * r)) & 0xf) |
-                                          ((uint32_t)((win >> (2 * r)) & 0xf) << 4);
-                    const bool valid = xcell && (y0 + r < g.cy);
-                    acc += valid ? lut[code] : 0u;
-                }
-            }
-            // 16-lane group sums = level-4 sums of four blocks.
-            acc += __shfl_xor(acc, 8, 16);
-            acc += __shfl_xor(acc, 4, 16);
-            acc += __shfl_xor(acc, 2, 16);
-            acc += __shfl_xor(acc, 1, 16);
-            if ((lane & 15) == 0 && x < g.cx) {
-                level4[block_key(g, x, y0, z)] = acc;
-            }
-        }
-        win_lo = win;
-        any_lo = any;
-        all_lo = all;
-        alle_lo = alle;
+    // Two plane buffers, each loaded one plane ahead of its use.  The
+    // prefetch is unconditional (the last one re-reads plane z1) so that the
+    // wait for a buffer never covers the other buffer's loads.
+    float a[kBlock + 1], b[kBlock + 1], ae, be;
+    ClassifyCarry c{0, 0, 0};
+    classify_load_plane(field, xl, w.y0, ymax, xe, ye, z0, a, ae);
+    int zz = z0;
+    for (;;) {
+        classify_load_plane(field, xl, w.y0, ymax, xe, ye, min(zz + 1, z1), b, be);
+        __builtin_amdgcn_sched_barrier(0);
+        classify_plane(w, g, T, a, ae, zz > z0, zz, c, level4, lut);
+        if (++zz > z1) break;
+        classify_load_plane(field, xl, w.y0, ymax, xe, ye, min(zz + 1, z1), a, ae);
+        __builtin_amdgcn_sched_barrier(0);
+        classify_plane(w, g, T, b, be, true, zz, c, level4, lut);
+        if (++zz > z1) break;
     }
 }
 
