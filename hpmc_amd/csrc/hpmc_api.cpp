@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -36,10 +37,10 @@ extern "C" __global__ __launch_bounds__(256) void hpmc_scan_reduce(const uint32_
 
 extern "C" __global__ __launch_bounds__(256) void hpmc_scan_scatter(const uint32_t* level4, uint32_t n,
                                                                     const uint2* tile_sums, uint32_t ntiles,
-                                                                    uint32_t* act_key, uint32_t* act_off,
-                                                                    hpmc::Totals* totals)
+                                                                    uint32_t* l4_off, uint2* active,
+                                                                    hpmc::Totals* totals, hpmc::Totals* host_totals)
 {
-    hpmc::scan_scatter_body(level4, n, tile_sums, ntiles, act_key, act_off, totals);
+    hpmc::scan_scatter_body(level4, n, tile_sums, ntiles, l4_off, active, totals, host_totals);
 }
 
 namespace {
@@ -81,8 +82,9 @@ struct HPMCHistoPyramid {
     uint64_t n_level4 = 0;
     uint2* tile_sums = nullptr;
     uint32_t n_tiles = 0;
-    uint32_t* act_key = nullptr;
-    uint32_t* act_off = nullptr;
+    uint32_t* l4_off = nullptr;
+    uint2* active = nullptr;
+    int emit_grid = 0;
     hpmc::Totals* d_totals = nullptr;
     hpmc::Totals* h_totals = nullptr;  // pinned
     hipEvent_t ev_totals = nullptr;
@@ -153,15 +155,16 @@ void free_device(HPMCHistoPyramid* h)
 {
     if (h->level4) (void)hipFree(h->level4);
     if (h->tile_sums) (void)hipFree(h->tile_sums);
-    if (h->act_key) (void)hipFree(h->act_key);
-    if (h->act_off) (void)hipFree(h->act_off);
+    if (h->l4_off) (void)hipFree(h->l4_off);
+    if (h->active) (void)hipFree(h->active);
     if (h->d_totals) (void)hipFree(h->d_totals);
     if (h->h_totals) (void)hipHostFree(h->h_totals);
     if (h->ev_totals) (void)hipEventDestroy(h->ev_totals);
     if (h->module) (void)hipModuleUnload(h->module);
     h->level4 = nullptr;
     h->tile_sums = nullptr;
-    h->act_key = h->act_off = nullptr;
+    h->l4_off = nullptr;
+    h->active = nullptr;
     h->d_totals = nullptr;
     h->h_totals = nullptr;
     h->ev_totals = nullptr;
@@ -227,6 +230,15 @@ std::string rtc_source(const std::string& snippet)
          "                                      ((float)k + 0.5f) * ipz));\n"
          "    }\n"
          "    __device__ __forceinline__ float row(int i, int j, int k) const { return at(i, j, k); }\n"
+         "    typedef float Plane;\n"
+         "    __device__ __forceinline__ uint32_t lane_index(int i, int j) const {\n"
+         "        return (uint32_t)i | ((uint32_t)j << 16);\n"
+         "    }\n"
+         "    __device__ __forceinline__ Plane plane(int k) const { return ((float)k + 0.5f) * ipz; }\n"
+         "    __device__ __forceinline__ float fetch(Plane pz, uint32_t idx) const {\n"
+         "        return HPMC_fetch(make_float3(((float)(idx & 0xffffu) + 0.5f) * ipx,\n"
+         "                                      ((float)(idx >> 16) + 0.5f) * ipy, pz));\n"
+         "    }\n"
          "};\n"
          "}\n"
          "HPMC_DEFINE_FIELD_KERNELS(rtc, (hpmc::SnippetField{g.ipx, g.ipy, g.ipz}))\n";
@@ -348,8 +360,16 @@ bool setup(HPMCHistoPyramid* h)
     HIP_TRY(h, hipMalloc(&h->level4, h->n_level4 * sizeof(uint32_t)));
     HIP_TRY(h, hipMemset(h->level4, 0, h->n_level4 * sizeof(uint32_t)));
     HIP_TRY(h, hipMalloc(&h->tile_sums, h->n_tiles * sizeof(uint2)));
-    HIP_TRY(h, hipMalloc(&h->act_key, h->n_level4 * sizeof(uint32_t)));
-    HIP_TRY(h, hipMalloc(&h->act_off, h->n_level4 * sizeof(uint32_t)));
+    HIP_TRY(h, hipMalloc(&h->l4_off, h->n_level4 * sizeof(uint32_t)));
+    HIP_TRY(h, hipMalloc(&h->active, h->n_level4 * sizeof(uint2)));
+    {
+        // persistent emit grid: as many 4-wave workgroups as fit at once
+        int dev = 0, cus = 0, per_cu = 0;
+        HIP_TRY(h, hipGetDevice(&dev));
+        HIP_TRY(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)hpmc_emit_volume, 256, 0));
+        h->emit_grid = (cus > 0 ? cus : 256) * (per_cu > 0 ? per_cu : 4);
+    }
     HIP_TRY(h, hipMalloc(&h->d_totals, sizeof(hpmc::Totals)));
     HIP_TRY(h, hipMemset(h->d_totals, 0, sizeof(hpmc::Totals)));
     HIP_TRY(h, hipHostMalloc(&h->h_totals, sizeof(hpmc::Totals), hipHostMallocDefault));
@@ -409,7 +429,7 @@ bool launch_scan(HPMCHistoPyramid* h)
     stage_mark(h, kStageScan);
     hipLaunchKernelGGL(hpmc_scan_reduce, dim3(h->n_tiles), dim3(256), 0, h->stream, h->level4, n, h->tile_sums);
     hipLaunchKernelGGL(hpmc_scan_scatter, dim3(h->n_tiles), dim3(256), 0, h->stream, h->level4, n,
-                       (const uint2*)h->tile_sums, h->n_tiles, h->act_key, h->act_off, h->d_totals);
+                       (const uint2*)h->tile_sums, h->n_tiles, h->l4_off, h->active, h->d_totals, h->h_totals);
     HIP_TRY(h, hipGetLastError());
     stage_mark(h, kStageScan);
     return true;
@@ -417,11 +437,9 @@ bool launch_scan(HPMCHistoPyramid* h)
 
 int emit_grid(const HPMCHistoPyramid* h)
 {
-    // persistent: about eight 256-thread workgroups per CU, never more than
-    // there are blocks in the pyramid level.
-    uint64_t g = 2048;
-    if (h->n_level4 < g) g = h->n_level4;
-    return (int)(g ? g : 1);
+    // persistent: never more waves than there are level-4 blocks
+    const uint64_t max_wg = (h->n_level4 + hpmc::kEmitWaves - 1) / hpmc::kEmitWaves;
+    return (int)std::min<uint64_t>((uint64_t)h->emit_grid, max_wg);
 }
 
 bool launch_emit(HPMCHistoPyramid* h, float* out, hpmc::CellEdge* out_ce, uint32_t capacity)
@@ -432,22 +450,19 @@ bool launch_emit(HPMCHistoPyramid* h, float* out, hpmc::CellEdge* out_ce, uint32
     stage_mark(h, kStageEmit);
     if (h->mode == kFieldVolume) {
         hipLaunchKernelGGL(hpmc_emit_volume, dim3(grid), dim3(256), 0, h->stream, g, h->threshold, fsrc,
-                           (const uint32_t*)h->act_key, (const uint32_t*)h->act_off, (const uint32_t*)h->level4,
-                           (const hpmc::Totals*)h->d_totals, out, out_ce, capacity,
+                           (const uint2*)h->active, (const hpmc::Totals*)h->d_totals, out, out_ce, capacity,
                            (const uint64_t*)h->constants->d_cases);
     } else {
         hpmc::Geom ga = g;
         float Ta = h->threshold;
         const void* fa = fsrc;
-        const uint32_t* ak = h->act_key;
-        const uint32_t* ao = h->act_off;
-        const uint32_t* l4 = h->level4;
+        const uint2* ac = h->active;
         const hpmc::Totals* tt = h->d_totals;
         float* o = out;
         hpmc::CellEdge* oc = out_ce;
         uint32_t cap = capacity;
         const uint64_t* cs = h->constants->d_cases;
-        void* args[] = {&ga, &Ta, &fa, &ak, &ao, &l4, &tt, &o, &oc, &cap, &cs};
+        void* args[] = {&ga, &Ta, &fa, &ac, &tt, &o, &oc, &cap, &cs};
         HIP_TRY(h, hipModuleLaunchKernel(h->k_emit, grid, 1, 1, 256, 1, 1, 0, h->stream, args, nullptr));
     }
     HIP_TRY(h, hipGetLastError());
@@ -641,9 +656,8 @@ void HPMCbuildHistopyramid(struct HPMCHistoPyramid* h, float threshold)
         h->broken = true;
         return;
     }
-    if (hipMemcpyAsync(h->h_totals, h->d_totals, sizeof(hpmc::Totals), hipMemcpyDeviceToHost, h->stream) !=
-            hipSuccess ||
-        hipEventRecord(h->ev_totals, h->stream) != hipSuccess) {
+    // the scan writes the totals straight into the pinned host copy
+    if (hipEventRecord(h->ev_totals, h->stream) != hipSuccess) {
         fail(h, HPMC_ERR_HIP, "failed to enqueue the count readback");
         h->broken = true;
         return;

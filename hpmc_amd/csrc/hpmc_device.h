@@ -97,6 +97,12 @@ __device__ __forceinline__ uint32_t block_key(const Geom& g, int x0, int y0, int
 
 // FP32 volume, x fastest; indices clamp to the lattice like a NEAREST/CLAMP
 // 3D texture (apps/texture3d/texture3d.cpp:135-139).
+//
+// Besides at(), a field offers plane-wise access for the kernels that stage
+// whole planes: lane_index(i, j) is a per-lane handle computed once,
+// plane(k) a wave-uniform handle per plane, fetch(plane, index) the sample.
+// For a volume these are a 32-bit in-plane offset and a plane pointer, so a
+// load is a scalar base plus a vector offset.
 struct VolumeField {
     const float* __restrict__ v;
     int sx, sy, sz;
@@ -114,6 +120,19 @@ struct VolumeField {
         const char* rp = (const char*)(v + ((size_t)k * sy + j) * sx);
         return *(const float*)(rp + i * 4u);
     }
+    typedef const char* Plane;
+    __device__ __forceinline__ uint32_t lane_index(int i, int j) const
+    {
+        i = min(max(i, 0), sx - 1);
+        j = min(max(j, 0), sy - 1);
+        return ((uint32_t)j * (uint32_t)sx + (uint32_t)i) * 4u;
+    }
+    __device__ __forceinline__ Plane plane(int k) const
+    {
+        k = min(max(k, 0), sz - 1);
+        return (const char*)(v + (size_t)k * sy * sx);
+    }
+    __device__ __forceinline__ float fetch(Plane p, uint32_t idx) const { return *(const float*)(p + idx); }
 };
 
 // ---------------------------------------------------------------------------
@@ -254,9 +273,10 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
 }
 
 // ---------------------------------------------------------------------------
-// scan, part 1: per 4096-block tile of the level-4 array, the sum of counts
+// scan, part 1: per 4096-entry tile of the level-4 array, the sum of counts
 // and of non-empty blocks.
 constexpr int kScanTile = 4096;
+constexpr int kScanPer = kScanTile / 256;   // entries per thread
 
 __device__ __forceinline__ uint2 block_reduce_u2(uint2 v, uint2* sh)
 {
@@ -277,34 +297,60 @@ __device__ __forceinline__ uint2 block_reduce_u2(uint2 v, uint2* sh)
     return r;
 }
 
+// The 16 entries of thread t of a tile, loaded as four 16-byte vectors
+// (all four in flight at once); entries past n read as zero.
+__device__ __forceinline__ void scan_load16(const uint32_t* __restrict__ level4, uint32_t n, uint32_t base,
+                                            uint32_t (&c)[kScanPer])
+{
+    if (base + kScanPer <= n) {
+        const uint4* p = (const uint4*)(level4 + base);
+#pragma unroll
+        for (int k = 0; k < kScanPer / 4; ++k) {
+            const uint4 q = p[k];
+            c[4 * k + 0] = q.x;
+            c[4 * k + 1] = q.y;
+            c[4 * k + 2] = q.z;
+            c[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) c[k] = (base + k < n) ? level4[base + k] : 0u;
+    }
+}
+
 __device__ __forceinline__ void scan_reduce_body(const uint32_t* __restrict__ level4, uint32_t n,
                                                  uint2* __restrict__ tile_sums)
 {
     __shared__ uint2 sh[16];
-    const uint32_t base = blockIdx.x * kScanTile;
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+    uint32_t c[kScanPer];
+    scan_load16(level4, n, base, c);
     uint2 s = make_uint2(0, 0);
-    for (uint32_t i = base + threadIdx.x; i < min(base + kScanTile, n); i += blockDim.x) {
-        uint32_t c = level4[i];
-        s.x += c;
-        s.y += c != 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        s.x += c[k];
+        s.y += c[k] != 0;
     }
     s = block_reduce_u2(s, sh);
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = s;
 }
 
 // scan, part 2: every tile re-reads the preceding tile sums (a few hundred),
-// scans its own 4096 entries and scatters the non-empty blocks with their
-// vertex offsets in Morton order.  Tile 0 also writes the totals.
+// scans its own 4096 entries and writes, for each non-empty level-4 block,
+// the exclusive vertex offset of its first vertex in Morton order (the
+// reference's key order).  Tile 0 also writes the totals, to device memory
+// and to the host-visible copy.
 __device__ __forceinline__ void scan_scatter_body(const uint32_t* __restrict__ level4, uint32_t n,
                                                   const uint2* __restrict__ tile_sums, uint32_t ntiles,
-                                                  uint32_t* __restrict__ act_key,
-                                                  uint32_t* __restrict__ act_off,
-                                                  Totals* __restrict__ totals)
+                                                  uint32_t* __restrict__ l4_off, uint2* __restrict__ active,
+                                                  Totals* __restrict__ totals, Totals* __restrict__ host_totals)
 {
     __shared__ uint2 sh[16];
     __shared__ uint2 wsum[16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+    uint32_t c[kScanPer];
+    scan_load16(level4, n, base, c);
     uint2 pre = make_uint2(0, 0);
     for (uint32_t i = threadIdx.x; i < blockIdx.x; i += blockDim.x) {
         pre.x += tile_sums[i].x;
@@ -319,27 +365,26 @@ __device__ __forceinline__ void scan_scatter_body(const uint32_t* __restrict__ l
         }
         tot = block_reduce_u2(tot, sh);
         if (threadIdx.x == 0) {
-            totals->vertices = tot.x;
-            totals->blocks = tot.y;
-            totals->overflow = 0;
+            Totals t;
+            t.vertices = tot.x;
+            t.blocks = tot.y;
+            t.overflow = 0;
+            t.pad = 0;
+            *totals = t;
+            if (host_totals) *host_totals = t;
         }
     }
-    // Each thread owns 16 consecutive entries of the tile.
-    constexpr int kPer = kScanTile / 256;
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kPer;
-    uint32_t c[kPer];
+    // vertex offsets (x) and non-empty block ranks (y), exclusive
     uint2 mine = make_uint2(0, 0);
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        c[k] = (base + k < n) ? level4[base + k] : 0u;
+    for (int k = 0; k < kScanPer; ++k) {
         mine.x += c[k];
         mine.y += c[k] != 0;
     }
-    // exclusive wave scan
     uint2 inc = mine;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        uint32_t ax = __shfl_up(inc.x, o), ay = __shfl_up(inc.y, o);
+        const uint32_t ax = __shfl_up(inc.x, o), ay = __shfl_up(inc.y, o);
         if (lane >= o) {
             inc.x += ax;
             inc.y += ay;
@@ -347,19 +392,16 @@ __device__ __forceinline__ void scan_scatter_body(const uint32_t* __restrict__ l
     }
     if (lane == 63) wsum[w] = inc;
     __syncthreads();
-    uint2 off = pre;
+    uint2 off = make_uint2(pre.x + inc.x - mine.x, pre.y + inc.y - mine.y);
     for (int i = 0; i < w; ++i) {
         off.x += wsum[i].x;
         off.y += wsum[i].y;
     }
-    (void)nw;
-    off.x += inc.x - mine.x;
-    off.y += inc.y - mine.y;
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
+    for (int k = 0; k < kScanPer; ++k) {
         if (c[k]) {
-            act_key[off.y] = base + k;
-            act_off[off.y] = off.x;
+            l4_off[base + k] = off.x;
+            active[off.y] = make_uint2(base + k, off.x);
             off.x += c[k];
             off.y += 1;
         }
@@ -367,118 +409,232 @@ __device__ __forceinline__ void scan_scatter_body(const uint32_t* __restrict__ l
 }
 
 // ---------------------------------------------------------------------------
-// emit: one workgroup of 256 threads per non-empty level-4 block, thread i
-// owning the cell at Morton index i of the block (levels 0..3 of the
-// pyramid below the block).
+// emit: a persistent grid; each wave takes non-empty level-4 blocks from the
+// compact list the scan wrote (Morton order, so consecutive items are
+// neighbours in the base image), one block per step.
+//
+// A block needs lattice planes z, z+1 (corners) and z+2 (forward
+// differences) over 18x18 samples.  They are loaded into registers one item
+// ahead (the key two items ahead) and staged in the wave's LDS region.  For
+// a block the 64 lanes classify its 256 cells (a 2x2 quad each, Morton
+// order), build the block's pyramid levels 0..3 in LDS (cells, quads, 16-
+// and 64-cell groups: four children each, ordered (0,0), (1,0), (0,1), (1,1)
+// as in the reference's traversal, shadergen.cpp:448-495), and then each
+// lane takes output vertices, descends the four levels to its cell and
+// writes the interpolated position and normal.
+constexpr int kEmitWaves = 4;
+constexpr int kPlaneLoads = (kBrickPlane + 63) / 64;   // 6 per lane per plane
+
+struct EmitWaveShared {
+    float plane[3][kBrickPlane];   // planes z, z+1, z+2 of the current block
+    uint16_t l0[256];              // per-cell vertex counts (Morton order)
+    uint16_t l1[64];               // per-quad sums
+    uint16_t l2[16];
+    uint16_t l3[4];
+    uint8_t code[256];
+};
+
 struct EmitShared {
     uint64_t cases[256];
-    float brick[kBrickSize];
-    uint16_t incl[256];      // inclusive vertex prefix over the block's cells
-    uint8_t code[256];
     uint8_t lut[256];
-    uint32_t wsum[4];
+    EmitWaveShared w[kEmitWaves];
 };
+
+// Orders this wave's LDS accesses around a phase change (each wave owns its
+// LDS region; no other wave takes part).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One level of the in-block descent: pick the child whose range holds k.
+__device__ __forceinline__ int descend4(const uint16_t* sums, uint32_t& k)
+{
+    const uint64_t q = *(const uint64_t*)sums;
+    const uint32_t s0 = (uint32_t)(q & 0xffff), s1 = (uint32_t)((q >> 16) & 0xffff);
+    const uint32_t s2 = (uint32_t)((q >> 32) & 0xffff);
+    int j = 0;
+    if (k >= s0) {
+        k -= s0;
+        j = 1;
+        if (k >= s1) {
+            k -= s1;
+            j = 2;
+            if (k >= s2) {
+                k -= s2;
+                j = 3;
+            }
+        }
+    }
+    return j;
+}
+
+// Block origin (x0, y0, z) of a level-4 key.
+__device__ __forceinline__ void block_origin(const Geom& g, uint32_t key, int& x0, int& y0, int& z)
+{
+    const int X0 = (int)morton_compact(key) << kBlockLog2;
+    const int Y0 = (int)morton_compact(key >> 1) << kBlockLog2;
+    const int tx = X0 / g.tsx2, ty = Y0 / g.tsy2;
+    z = ty * g.tiles_x + tx;
+    x0 = X0 - tx * g.tsx2;
+    y0 = Y0 - ty * g.tsy2;
+}
+
+template <class F>
+__device__ __forceinline__ void emit_fetch_block(const F& field, const Geom& g, uint32_t key, int lane,
+                                                 float (&r)[3][kPlaneLoads])
+{
+    int x0, y0, z;
+    block_origin(g, key, x0, y0, z);
+    uint32_t idx[kPlaneLoads];
+#pragma unroll
+    for (int k = 0; k < kPlaneLoads; ++k) {
+        const int i = min(lane + 64 * k, kBrickPlane - 1);
+        const int ky = i / kBrick, kx = i - ky * kBrick;
+        idx[k] = field.lane_index(x0 + kx, y0 + ky);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const typename F::Plane pl = field.plane(z + q);
+#pragma unroll
+        for (int k = 0; k < kPlaneLoads; ++k) r[q][k] = field.fetch(pl, idx[k]);
+    }
+}
 
 template <class F>
 __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T,
-                                          const uint32_t* __restrict__ act_key,
-                                          const uint32_t* __restrict__ act_off,
-                                          const uint32_t* __restrict__ level4,
-                                          const Totals* __restrict__ totals,
+                                          const uint2* __restrict__ active, const Totals* __restrict__ totals,
                                           float* __restrict__ out, CellEdge* __restrict__ out_ce,
                                           uint32_t capacity, EmitShared& sh)
 {
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const uint32_t nblocks = __builtin_amdgcn_readfirstlane(totals->blocks);
+    const uint32_t stride = gridDim.x * kEmitWaves;
+    uint32_t item = __builtin_amdgcn_readfirstlane(blockIdx.x * kEmitWaves + wv);
+    if (item >= nblocks) return;
+
+    EmitWaveShared& ws = sh.w[wv];
     const uint64_t* cases = sh.cases;
     const uint8_t* lut = sh.lut;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, w = tid >> 6;
-    const uint32_t nblocks = totals->blocks;
-    const uint32_t lx = morton_compact((uint32_t)tid), ly = morton_compact((uint32_t)tid >> 1);
+    // this lane's 2x2 quad of cells: Morton indices 4*lane .. 4*lane+3
+    const int qx = (int)morton_compact((uint32_t)lane) * 2;
+    const int qy = (int)morton_compact((uint32_t)lane >> 1) * 2;
 
-    for (uint32_t bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
-        const uint32_t key = act_key[bi];
-        const uint32_t base = act_off[bi];
-        const uint32_t stored = level4[key];
-        const int X0 = (int)morton_compact(key) << kBlockLog2;
-        const int Y0 = (int)morton_compact(key >> 1) << kBlockLog2;
-        const int tx = X0 / g.tsx2, ty = Y0 / g.tsy2;
-        const int z = ty * g.tiles_x + tx;
-        const int x0 = X0 - tx * g.tsx2, y0 = Y0 - ty * g.tsy2;
+    uint2 cur = active[item];
+    cur.x = __builtin_amdgcn_readfirstlane(cur.x);
+    cur.y = __builtin_amdgcn_readfirstlane(cur.y);
+    float r[3][kPlaneLoads];
+    emit_fetch_block(field, g, cur.x, lane, r);
+    uint32_t next_item = item + stride;
+    uint2 nxt = make_uint2(0, 0);
+    if (next_item < nblocks) nxt = active[next_item];
 
-        for (int i = tid; i < kBrickSize; i += 256) {
-            const int kz = i / kBrickPlane, r = i - kz * kBrickPlane;
-            const int ky = r / kBrick, kx = r - ky * kBrick;
-            sh.brick[i] = field.at(x0 + kx, y0 + ky, z + kz);
-        }
-        __syncthreads();
-
-        // cell code and count
-        const int cxl = (int)lx, cyl = (int)ly;
-        uint32_t code = 0;
+    for (;;) {
+        const uint32_t key = cur.x, base = cur.y;
+        int x0, y0, z;
+        block_origin(g, key, x0, y0, z);
+        wave_lds_sync();
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const float v = sh.brick[((b >> 2) * kBrick + cyl + ((b >> 1) & 1)) * kBrick + cxl + (b & 1)];
-            code |= (v < T ? 1u : 0u) << b;
-        }
-        const bool valid = (x0 + cxl < g.cx) && (y0 + cyl < g.cy);
-        const uint32_t cnt = valid ? lut[code] : 0u;
-        // inclusive scan over the 256 cells (Morton order)
-        uint32_t inc = cnt;
+        for (int q = 0; q < 3; ++q)
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t a = __shfl_up(inc, o);
-            if (lane >= o) inc += a;
-        }
-        if (lane == 63) sh.wsum[w] = inc;
-        __syncthreads();
-        for (int i = 0; i < w; ++i) inc += sh.wsum[i];
-        sh.incl[tid] = (uint16_t)inc;
-        sh.code[tid] = (uint8_t)code;
-        __syncthreads();
-        uint32_t nv = sh.incl[255];
-        if (nv > stored) nv = stored;
-
-        for (uint32_t v = tid; v < nv; v += 256) {
-            // descend: first cell whose inclusive prefix exceeds v
-            int lo = 0;
-#pragma unroll
-            for (int step = 128; step > 0; step >>= 1) {
-                if (sh.incl[lo + step - 1] <= v) lo += step;
+            for (int k = 0; k < kPlaneLoads; ++k) {
+                const int i = lane + 64 * k;
+                if (i < kBrickPlane) ws.plane[q][i] = r[q][k];
             }
-            const int c = lo;
-            const uint32_t before = c ? sh.incl[c - 1] : 0u;
-            const uint32_t slot = v - before;
-            const uint32_t ccode = sh.code[c];
-            const uint32_t e = (uint32_t)(cases[ccode] >> (4 + 4 * slot)) & 0xfu;
+        // prefetch the next item's planes and the key after it
+        const bool more = next_item < nblocks;
+        uint2 after = make_uint2(0, 0);
+        if (more) {
+            nxt.x = __builtin_amdgcn_readfirstlane(nxt.x);
+            nxt.y = __builtin_amdgcn_readfirstlane(nxt.y);
+            emit_fetch_block(field, g, nxt.x, lane, r);
+            if (next_item + stride < nblocks) after = active[next_item + stride];
+        }
+        wave_lds_sync();
+
+        const float* P0 = ws.plane[0];
+        const float* P1 = ws.plane[1];
+        const float* P2 = ws.plane[2];
+        const bool qvx0 = x0 + qx < g.cx, qvx1 = x0 + qx + 1 < g.cx;
+        const bool qvy0 = y0 + qy < g.cy, qvy1 = y0 + qy + 1 < g.cy;
+
+        // classify the quad: corners (qx..qx+2, qy..qy+2) of planes z, z+1
+        uint32_t bits0 = 0, bits1 = 0;   // 3x3 inside bits per plane, bit = dx + 3*dy
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int i = (qy + dy) * kBrick + qx + dx;
+                bits0 |= (P0[i] < T ? 1u : 0u) << (dx + 3 * dy);
+                bits1 |= (P1[i] < T ? 1u : 0u) << (dx + 3 * dy);
+            }
+        uint32_t codes4 = 0, cnt4 = 0;
+        uint64_t l0q = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int cxq = q & 1, cyq = q >> 1;
+            const int sh0 = cxq + 3 * cyq;
+            const uint32_t lo = ((bits0 >> sh0) & 3u) | (((bits0 >> (sh0 + 3)) & 3u) << 2);
+            const uint32_t hi = ((bits1 >> sh0) & 3u) | (((bits1 >> (sh0 + 3)) & 3u) << 2);
+            const uint32_t code = lo | (hi << 4);
+            const bool valid = (cxq ? qvx1 : qvx0) && (cyq ? qvy1 : qvy0);
+            const uint32_t c = lut[code];
+            const uint32_t cv = valid ? c : 0u;
+            codes4 |= code << (8 * q);
+            cnt4 += cv;
+            l0q |= (uint64_t)cv << (16 * q);
+        }
+        // pyramid levels 1..3 of the block
+        uint32_t g4 = cnt4 + __shfl_xor(cnt4, 1);
+        g4 += __shfl_xor(g4, 2);
+        uint32_t g16 = g4 + __shfl_xor(g4, 4);
+        g16 += __shfl_xor(g16, 8);
+        uint32_t g64 = g16 + __shfl_xor(g16, 16);
+        g64 += __shfl_xor(g64, 32);
+        *(uint64_t*)&ws.l0[4 * lane] = l0q;
+        *(uint32_t*)&ws.code[4 * lane] = codes4;
+        ws.l1[lane] = (uint16_t)cnt4;
+        if ((lane & 3) == 0) ws.l2[lane >> 2] = (uint16_t)g4;
+        if ((lane & 15) == 0) ws.l3[lane >> 4] = (uint16_t)g16;
+        const uint32_t nv = g64;   // whole-block total (every lane)
+        wave_lds_sync();
+
+        for (uint32_t v = lane; v < nv; v += 64) {
+            uint32_t k = v;
+            int c = descend4(ws.l3, k);
+            c = 4 * c + descend4(ws.l2 + 4 * c, k);
+            c = 4 * c + descend4(ws.l1 + 4 * c, k);
+            c = 4 * c + descend4(ws.l0 + 4 * c, k);
+            const uint32_t slot = k;
+            const uint32_t e = (uint32_t)(cases[ws.code[c]] >> (4 + 4 * slot)) & 0xfu;
             const int axis = (int)(e >> 2);
             const int s0 = (int)(e & 1), s1 = (int)((e >> 1) & 1);
             const int ccx = (int)morton_compact((uint32_t)c), ccy = (int)morton_compact((uint32_t)c >> 1);
             int ax = ccx, ay = ccy, az = 0;
-            if (axis == 0) { ay += s0; az += s1; }
-            else if (axis == 1) { ax += s0; az += s1; }
+            if (axis == 0) { ay += s0; az = s1; }
+            else if (axis == 1) { ax += s0; az = s1; }
             else { ax += s0; ay += s1; }
-            const int bx = ax + (axis == 0), by = ay + (axis == 1), bz = az + (axis == 2);
-            const float* B = sh.brick;
-            const float va = B[(az * kBrick + ay) * kBrick + ax];
-            const float vb = B[(bz * kBrick + by) * kBrick + bx];
-            const float nax = B[(az * kBrick + ay) * kBrick + ax + 1];
-            const float nay = B[(az * kBrick + ay + 1) * kBrick + ax];
-            const float naz = B[((az + 1) * kBrick + ay) * kBrick + ax];
-            const float nbx = B[(bz * kBrick + by) * kBrick + bx + 1];
-            const float nby = B[(bz * kBrick + by + 1) * kBrick + bx];
-            const float nbz = B[((bz + 1) * kBrick + by) * kBrick + bx];
+            const int bx_ = ax + (axis == 0), by_ = ay + (axis == 1), bz_ = az + (axis == 2);
+            const float* Pa = az ? P1 : P0;
+            const float* Pan = az ? P2 : P1;
+            const float* Pb = bz_ ? P1 : P0;
+            const float* Pbn = bz_ ? P2 : P1;
+            const int ia = ay * kBrick + ax, ib = by_ * kBrick + bx_;
+            const float va = Pa[ia], vb = Pb[ib];
+            const float nax = Pa[ia + 1], nay = Pa[ia + kBrick], naz = Pan[ia];
+            const float nbx_ = Pb[ib + 1], nby_ = Pb[ib + kBrick], nbz_ = Pbn[ib];
             const float t = (va - T) / (va - vb);
             const float u = 1.0f - t;
             const float gx = (float)(x0 + ax), gy = (float)(y0 + ay), gz = (float)(z + az);
             const uint32_t o = base + v;
             if (o < capacity) {
-                float* dst = out + (size_t)o * 6;
-                dst[0] = (T - (nax * u + nbx * t)) * g.scx;
-                dst[1] = (T - (nay * u + nby * t)) * g.scy;
-                dst[2] = (T - (naz * u + nbz * t)) * g.scz;
-                dst[3] = (gx + (axis == 0 ? t : 0.0f)) * g.scx;
-                dst[4] = (gy + (axis == 1 ? t : 0.0f)) * g.scy;
-                dst[5] = (gz + (axis == 2 ? t : 0.0f)) * g.scz;
+                float2* dst = (float2*)(out + (size_t)o * 6);
+                dst[0] = make_float2((T - (nax * u + nbx_ * t)) * g.scx, (T - (nay * u + nby_ * t)) * g.scy);
+                dst[1] = make_float2((T - (naz * u + nbz_ * t)) * g.scz, (gx + (axis == 0 ? t : 0.0f)) * g.scx);
+                dst[2] = make_float2((gy + (axis == 1 ? t : 0.0f)) * g.scy, (gz + (axis == 2 ? t : 0.0f)) * g.scz);
                 if (out_ce) {
                     CellEdge ce;
                     ce.cell = (uint32_t)(x0 + ccx) + (uint32_t)g.cx * ((uint32_t)(y0 + ccy) + (uint32_t)g.cy * (uint32_t)z);
@@ -487,7 +643,10 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
                 }
             }
         }
-        __syncthreads();
+        if (!more) break;
+        cur = nxt;
+        nxt = after;
+        next_item += stride;
     }
 }
 
@@ -523,16 +682,14 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
         hpmc::classify_body(MAKE_FIELD, g, T, level4, zseg, lut);                             \
     }                                                                                         \
     extern "C" __global__ __launch_bounds__(256) void hpmc_emit_##SUFFIX(                      \
-        hpmc::Geom g, float T, const void* fsrc, const uint32_t* act_key,                     \
-        const uint32_t* act_off, const uint32_t* level4, const hpmc::Totals* totals,          \
+        hpmc::Geom g, float T, const void* fsrc, const uint2* active, const hpmc::Totals* totals, \
         float* out, hpmc::CellEdge* out_ce, uint32_t capacity, const uint64_t* cases)         \
     {                                                                                         \
         __shared__ hpmc::EmitShared sh;                                                       \
         sh.cases[threadIdx.x] = cases[threadIdx.x];                                           \
         sh.lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);                           \
         __syncthreads();                                                                      \
-        hpmc::emit_body(MAKE_FIELD, g, T, act_key, act_off, level4, totals, out, out_ce,      \
-                        capacity, sh);                                                        \
+        hpmc::emit_body(MAKE_FIELD, g, T, active, totals, out, out_ce, capacity, sh);         \
     }                                                                                         \
     extern "C" __global__ __launch_bounds__(256) void hpmc_codes_##SUFFIX(                     \
         hpmc::Geom g, float T, const void* fsrc, uint8_t* codes)                              \
