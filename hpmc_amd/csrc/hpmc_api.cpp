@@ -229,7 +229,6 @@ std::string rtc_source(const std::string& snippet)
          "        return HPMC_fetch(make_float3(((float)i + 0.5f) * ipx, ((float)j + 0.5f) * ipy,\n"
          "                                      ((float)k + 0.5f) * ipz));\n"
          "    }\n"
-         "    __device__ __forceinline__ float row(int i, int j, int k) const { return at(i, j, k); }\n"
          "    typedef float Plane;\n"
          "    __device__ __forceinline__ uint32_t lane_index(int i, int j) const {\n"
          "        return (uint32_t)i | ((uint32_t)j << 16);\n"
@@ -297,11 +296,24 @@ bool compile_snippet(HPMCHistoPyramid* h)
     return true;
 }
 
-int choose_zseg(const hpmc::Geom& g)
+// Classification launch shape: one workgroup of ceil(X/64) waves (at most
+// 16) per band of 16 cell rows and zseg slices.  zseg is picked so the
+// workgroups fill the device about once (each extra segment re-reads one
+// plane), between 8 and 64 slices.
+int classify_waves(const hpmc::Geom& g)
 {
-    const int64_t per_slice = (int64_t)((g.cx + 63) / 64) * ((g.cy + hpmc::kBlock - 1) / hpmc::kBlock);
-    const int64_t target_waves = 8192;
-    int64_t z = (g.cz * per_slice + target_waves - 1) / target_waves;
+    int nw = (g.sx + 63) / 64;
+    if (nw > hpmc::kClassifyMaxWaves) nw = hpmc::kClassifyMaxWaves;
+    return nw;
+}
+
+int choose_zseg(const hpmc::Geom& g, int resident_wgs)
+{
+    const int nw = classify_waves(g);
+    const int64_t wgx = (g.sx + 64 * nw - 1) / (64 * nw);
+    const int64_t bands = wgx * ((g.cy + hpmc::kBlock - 1) / hpmc::kBlock);
+    int64_t target = resident_wgs > 0 ? resident_wgs : 1024;
+    int64_t z = (g.cz * bands + target - 1) / target;
     if (z < 8) z = 8;
     if (z > 64) z = 64;
     return (int)z;
@@ -353,7 +365,14 @@ bool setup(HPMCHistoPyramid* h)
     g.ipx = 1.0f / (float)h->size[0];
     g.ipy = 1.0f / (float)h->size[1];
     g.ipz = 1.0f / (float)h->size[2];
-    h->zseg = choose_zseg(g);
+    {
+        int dev = 0, cus = 0, per_cu = 0;
+        HIP_TRY(h, hipGetDevice(&dev));
+        HIP_TRY(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)hpmc_classify_volume,
+                                                                 64 * classify_waves(g), 0));
+        h->zseg = choose_zseg(g, cus * (per_cu > 0 ? per_cu : 1));
+    }
     const uint64_t side = (uint64_t)1 << g.blocks_log2;
     h->n_level4 = side * side;
     h->n_tiles = (uint32_t)((h->n_level4 + hpmc::kScanTile - 1) / hpmc::kScanTile);
@@ -400,13 +419,16 @@ const void* field_source(HPMCHistoPyramid* h) { return h->mode == kFieldVolume ?
 bool launch_classify(HPMCHistoPyramid* h, float T)
 {
     const hpmc::Geom& g = h->geom;
-    const int64_t waves = (int64_t)((g.cx + 63) / 64) * ((g.cy + hpmc::kBlock - 1) / hpmc::kBlock) *
-                          ((g.cz + h->zseg - 1) / h->zseg);
-    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    const int nw = classify_waves(g);
+    const int64_t wgx = (g.sx + 64 * nw - 1) / (64 * nw);
+    // the XCD-aware tile mapping needs the grid padded to a multiple of 8
+    const int64_t tiles = wgx * ((g.cy + hpmc::kBlock - 1) / hpmc::kBlock) * ((g.cz + h->zseg - 1) / h->zseg);
+    const unsigned blocks = (unsigned)(((tiles + 7) / 8) * 8);
+    const unsigned threads = 64u * (unsigned)nw;
     const void* fsrc = field_source(h);
     stage_mark(h, kStageClassify);
     if (h->mode == kFieldVolume) {
-        hipLaunchKernelGGL(hpmc_classify_volume, dim3(blocks), dim3(256), 0, h->stream, g, T, fsrc, h->level4,
+        hipLaunchKernelGGL(hpmc_classify_volume, dim3(blocks), dim3(threads), 0, h->stream, g, T, fsrc, h->level4,
                            h->zseg, (const uint64_t*)h->constants->d_cases);
     } else {
         hpmc::Geom ga = g;
@@ -416,7 +438,7 @@ bool launch_classify(HPMCHistoPyramid* h, float T)
         int zs = h->zseg;
         const uint64_t* cs = h->constants->d_cases;
         void* args[] = {&ga, &Ta, &fa, &l4, &zs, &cs};
-        HIP_TRY(h, hipModuleLaunchKernel(h->k_classify, blocks, 1, 1, 256, 1, 1, 0, h->stream, args, nullptr));
+        HIP_TRY(h, hipModuleLaunchKernel(h->k_classify, blocks, 1, 1, threads, 1, 1, 0, h->stream, args, nullptr));
     }
     HIP_TRY(h, hipGetLastError());
     stage_mark(h, kStageClassify);

@@ -113,14 +113,9 @@ struct VolumeField {
         k = min(max(k, 0), sz - 1);
         return v[((size_t)k * sy + j) * sx + i];
     }
-    // In-range load without clamping (classification rows): a row pointer
-    // that is uniform across the wave plus the lane's x offset.
-    __device__ __forceinline__ float row(unsigned i, int j, int k) const
-    {
-        const char* rp = (const char*)(v + ((size_t)k * sy + j) * sx);
-        return *(const float*)(rp + i * 4u);
-    }
-    typedef const char* Plane;
+    // Plane-wise access goes through a buffer resource per plane (a scalar
+    // 128-bit descriptor) and 32-bit per-lane byte offsets.
+    typedef __amdgpu_buffer_rsrc_t Plane;
     __device__ __forceinline__ uint32_t lane_index(int i, int j) const
     {
         i = min(max(i, 0), sx - 1);
@@ -130,112 +125,145 @@ struct VolumeField {
     __device__ __forceinline__ Plane plane(int k) const
     {
         k = min(max(k, 0), sz - 1);
-        return (const char*)(v + (size_t)k * sy * sx);
+        const float* base = v + (size_t)k * sy * sx;
+        return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, sx * sy * 4, 0x00020000);
     }
-    __device__ __forceinline__ float fetch(Plane p, uint32_t idx) const { return *(const float*)(p + idx); }
+    __device__ __forceinline__ float fetch(Plane p, uint32_t idx) const
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p, idx, 0, 0));
+    }
 };
 
 // ---------------------------------------------------------------------------
-// classify: one wave per (64 x 16 cells) x zseg-slice column.
+// classify: one workgroup per (row band of 16 cells) x (zseg slices); its
+// waves sit side by side along x, 64 lattice columns each, so a workgroup of
+// ceil(X/64) waves covers whole lattice rows (up to 1024 columns; wider
+// lattices take several workgroups per band, joined by one extra column).
 //
-// Each plane is read once as 17 rows of 64 samples plus the 17 samples of
-// the next column (one vector load), turned into 64-bit ballot masks of
-// (f < T), and the codes of a whole cell row come out of four masks.  Loads
-// use clamped, always-valid addresses so a plane's 18 loads issue back to
-// back, and the next plane is loaded while the current one is classified.
-// Slices whose two planes are uniformly outside or inside cannot hold a
-// surface cell and are skipped with scalar tests.  Counts are summed per
-// lane over the 16 rows of a block and across 16-lane groups, giving four
-// level-4 sums per slice.
-template <class F>
-__device__ __forceinline__ void classify_load_plane(const F& field, unsigned xl, int y0, int ymax, unsigned xe,
-                                                    int ye, int zz, float (&v)[kBlock + 1], float& ve)
-{
-#pragma unroll
-    for (int r = 0; r <= kBlock; ++r) v[r] = field.row(xl, min(y0 + r, ymax), zz);
-    ve = field.row(xe, ye, zz);
-}
+// Each plane is read once as 17 rows of 64 samples per wave and turned into
+// 64-bit ballot masks of (f < T); the codes of a whole cell row come out of
+// four masks.  The sample right of a wave's last lane belongs to the next
+// wave, whose lane-0 bits are passed through LDS, so no extra column is
+// loaded inside a workgroup.  Loads use clamped, always-valid addresses so a
+// plane's loads issue back to back, and the next plane is in flight while
+// the current one is classified.  Slices whose two planes are uniformly
+// outside or inside cannot hold a surface cell and are skipped with scalar
+// tests.  Counts are summed per lane over the 16 rows of a block and across
+// 16-lane groups, giving four level-4 sums per wave and slice.
+constexpr int kClassifyMaxWaves = 16;
 
-// Per-wave state carried from plane to plane.
-struct ClassifyCarry {
-    uint64_t win;   // per lane: 2-bit windows of the previous plane's 17 rows
-    uint64_t any;   // previous plane: some sample inside
-    uint64_t all;   // previous plane: every sample inside
-};
+template <class F>
+__device__ __forceinline__ void classify_load_plane(const F& field, const uint32_t (&roff)[kBlock + 1],
+                                                    uint32_t eoff, bool load_e, int zz, float (&v)[kBlock + 1],
+                                                    float& ve)
+{
+    const typename F::Plane pl = field.plane(zz);
+#pragma unroll
+    for (int r = 0; r <= kBlock; ++r) v[r] = field.fetch(pl, roff[r]);
+    if (load_e) ve = field.fetch(pl, eoff);
+}
 
 struct ClassifyWave {
     int lane, x0, y0, x, sy, cx, cy;
-    bool xin, xcell, ein, lane_e;
-    uint64_t lane63;
+    bool xin, xcell, has_right, lane_e;
 };
 
-// Classifies plane `zz` against the carried plane zz-1 (when have_lo) and
-// writes the four level-4 sums of slice zz-1.
-__device__ __forceinline__ void classify_plane(const ClassifyWave& w, const Geom& g, float T,
-                                               const float (&v)[kBlock + 1], float ve, bool have_lo, int zz,
-                                               ClassifyCarry& c, uint32_t* __restrict__ level4,
-                                               const uint8_t* __restrict__ lut)
+// Per-wave state carried from plane to plane.
+struct ClassifyCarry {
+    uint32_t col;   // per lane: previous plane's inside bits of this lane's column, bit r = row r
+    uint64_t eb;    // previous plane: inside bits of the column right of the wave (uniform)
+    uint64_t any;   // previous plane: some lane's column has an inside sample (uniform)
+    uint64_t all;   // previous plane: every sample inside, incl. the right column (uniform)
+};
+
+// The 17 rows' inside bits of this lane's column (bit r = row r).
+__device__ __forceinline__ uint32_t classify_column(const ClassifyWave& w, float T, const float (&v)[kBlock + 1])
 {
-    // bit r of eb: the sample at (x0+64, y0+r) is inside
-    const uint64_t eb = __ballot(w.lane_e && ve < T);
-    uint64_t win = 0, any = eb, all = ~0ull;
+    uint32_t col = 0;
 #pragma unroll
-    for (int r = 0; r <= kBlock; ++r) {
-        const uint64_t m = __ballot(w.xin && (w.y0 + r < w.sy) && v[r] < T);
-        any |= m;
-        all &= m;
-        const uint64_t e = (eb >> r) & 1ull;
-        const uint64_t bits = ((m >> w.lane) & 3ull) | ((e & w.lane63) << 1);
-        win |= bits << (2 * r);
+    for (int r = 0; r <= kBlock; ++r) col |= ((w.xin && (w.y0 + r < w.sy) && v[r] < T) ? 1u : 0u) << r;
+    return col;
+}
+
+// Interleaves a column's bits with its right neighbour's: bit 2r = own
+// row r, bit 2r+1 = neighbour row r.  Bits 2r..2r+3 are then the four
+// corners of cell row r on this plane in HPMC code order (x + 2y).
+__device__ __forceinline__ uint64_t classify_window(uint32_t col, uint32_t right)
+{
+    const uint64_t a = (uint64_t)morton_spread(col & 0xffffu) | ((uint64_t)((col >> 16) & 1u) << 32);
+    const uint64_t b = (uint64_t)morton_spread(right & 0xffffu) | ((uint64_t)((right >> 16) & 1u) << 32);
+    return a | (b << 1);
+}
+
+// Classifies slice zz-1 from the column bits of planes zz-1 (clo) and zz
+// (chi): 16 cell rows of 64 cells, summed into four level-4 blocks.  Only
+// called when the two planes are mixed.
+__device__ __forceinline__ uint32_t classify_slice_counts(const ClassifyWave& w, uint32_t clo, uint32_t chi,
+                                                          uint64_t eb_lo, uint64_t eb_hi,
+                                                          const uint8_t* __restrict__ lut)
+{
+    uint32_t rlo = __shfl_down(clo, 1), rhi = __shfl_down(chi, 1);
+    if (w.lane == 63) {
+        rlo = (uint32_t)eb_lo;
+        rhi = (uint32_t)eb_hi;
     }
-    // all-inside needs every row's +1 column inside as well
-    const uint64_t erows = (1ull << (kBlock + 1)) - 1ull;
-    if (!w.ein || (eb & erows) != erows) all = 0;
-    if (have_lo) {
-        const int z = zz - 1;
-        uint32_t acc = 0;
-        const bool uniform_out = (c.any | any) == 0ull;
-        const bool uniform_in = (c.all & all) == ~0ull;
-        if (!(uniform_out || uniform_in)) {
+    const uint64_t wlo = classify_window(clo, rlo), whi = classify_window(chi, rhi);
+    uint32_t acc = 0;
 #pragma unroll
-            for (int r = 0; r < kBlock; ++r) {
-                const uint32_t code = (uint32_t)((c.win >> (2 * r)) & 0xf) | ((uint32_t)((win >> (2 * r)) & 0xf) << 4);
-                const bool valid = w.xcell && (w.y0 + r < w.cy);
-                acc += valid ? lut[code] : 0u;
-            }
-        }
-        // 16-lane group sums = level-4 sums of four blocks.
-        acc += __shfl_xor(acc, 8, 16);
-        acc += __shfl_xor(acc, 4, 16);
-        acc += __shfl_xor(acc, 2, 16);
-        acc += __shfl_xor(acc, 1, 16);
-        if ((w.lane & 15) == 0 && w.x < w.cx) level4[block_key(g, w.x, w.y0, z)] = acc;
+    for (int r = 0; r < kBlock; ++r) {
+        const uint32_t code = (uint32_t)((wlo >> (2 * r)) & 0xf) | ((uint32_t)((whi >> (2 * r)) & 0xf) << 4);
+        const bool valid = w.xcell && (w.y0 + r < w.cy);
+        const uint32_t cnt = lut[code];
+        acc += valid ? cnt : 0u;
     }
-    c.win = win;
-    c.any = any;
-    c.all = all;
+    // 16-lane group sums = level-4 sums of four blocks.
+    acc += __shfl_xor(acc, 8, 16);
+    acc += __shfl_xor(acc, 4, 16);
+    acc += __shfl_xor(acc, 2, 16);
+    acc += __shfl_xor(acc, 1, 16);
+    return acc;
+}
+
+// Workgroup -> (x band, y band, z segment).  Workgroups are dealt to the
+// eight XCDs round robin; the mapping gives each XCD a contiguous run of
+// y bands, so workgroups resident together on one XCD are y-neighbours and
+// share their halo rows in that XCD's L2.
+__device__ __forceinline__ void classify_tile(const Geom& g, int zseg, int wgx, int& bx, int& py, int& pz)
+{
+    const int npy = (g.cy + kBlock - 1) >> kBlockLog2;
+    const int npz = (g.cz + zseg - 1) / zseg;
+    const int total = wgx * npy * npz;
+    const int b = (int)blockIdx.x;
+    const int per_xcd = (total + 7) >> 3;
+    const int t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= total) {
+        bx = -1;
+        return;
+    }
+    bx = t % wgx;
+    const int rest = t / wgx;
+    py = rest % npy;
+    pz = rest / npy;
 }
 
 template <class F>
 __device__ __forceinline__ void classify_body(const F& field, const Geom& g, float T,
                                               uint32_t* __restrict__ level4, int zseg,
-                                              const uint8_t* __restrict__ lut)
+                                              const uint8_t* __restrict__ lut, uint32_t (*xch)[kClassifyMaxWaves])
 {
     const int lane = threadIdx.x & 63;
-    // wave-uniform: keeps the column indices, plane loop and row pointers in
-    // scalar registers
-    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    const int npx = (g.cx + 63) >> 6;
-    const int npy = (g.cy + kBlock - 1) >> kBlockLog2;
-    const int npz = (g.cz + zseg - 1) / zseg;
-    if (wave >= npx * npy * npz) return;
-    const int px = wave % npx;
-    const int rest = wave / npx;
-    const int py = rest % npy;
-    const int pz = rest / npy;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = blockDim.x >> 6;
+    const int wgx = (g.sx + 64 * nw - 1) / (64 * nw);    // workgroups across x
+    int bx, py, pz;
+    classify_tile(g, zseg, wgx, bx, py, pz);
+    if (bx < 0) return;   // whole workgroup
+    bx = __builtin_amdgcn_readfirstlane(bx);
+    py = __builtin_amdgcn_readfirstlane(py);
+    pz = __builtin_amdgcn_readfirstlane(pz);
     ClassifyWave w;
     w.lane = lane;
-    w.x0 = px << 6;
+    w.x0 = (bx * nw + wv) << 6;
     w.y0 = py << kBlockLog2;
     w.x = w.x0 + lane;
     w.sy = g.sy;
@@ -243,31 +271,62 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
     w.cy = g.cy;
     w.xin = w.x < g.sx;
     w.xcell = w.x < g.cx;
-    w.ein = w.x0 + 64 < g.sx;
-    w.lane_e = w.ein && lane <= kBlock && (w.y0 + lane < g.sy);
-    w.lane63 = lane == 63 ? 1ull : 0ull;
+    w.has_right = w.x0 + 64 < g.sx;
+    const bool last = wv == nw - 1;
+    // right column from the next workgroup (wave-uniform: a scalar branch)
+    const bool load_e = __builtin_amdgcn_readfirstlane((int)(last && w.has_right)) != 0;
+    w.lane_e = load_e && lane <= kBlock && (w.y0 + lane < g.sy);
     const int z0 = pz * zseg;
     const int z1 = min(z0 + zseg, g.cz);   // cell slices [z0, z1): planes z0..z1
-    const unsigned xl = (unsigned)min(w.x, g.sx - 1);
-    const unsigned xe = (unsigned)min(w.x0 + 64, g.sx - 1);
-    const int ye = min(w.y0 + min(lane, kBlock), g.sy - 1);
-    const int ymax = g.sy - 1;
+    // per-lane plane offsets of the 17 rows (clamped) and of the column right
+    // of the workgroup
+    uint32_t roff[kBlock + 1];
+#pragma unroll
+    for (int r = 0; r <= kBlock; ++r) roff[r] = field.lane_index(w.x, w.y0 + r);
+    const uint32_t eoff = field.lane_index(w.x0 + 64, w.y0 + min(lane, kBlock));
+    const uint64_t erows = (1ull << (kBlock + 1)) - 1ull;
 
-    // Two plane buffers, each loaded one plane ahead of its use.  The
-    // prefetch is unconditional (the last one re-reads plane z1) so that the
-    // wait for a buffer never covers the other buffer's loads.
-    float a[kBlock + 1], b[kBlock + 1], ae, be;
-    ClassifyCarry c{0, 0, 0};
-    classify_load_plane(field, xl, w.y0, ymax, xe, ye, z0, a, ae);
-    int zz = z0;
-    for (;;) {
-        classify_load_plane(field, xl, w.y0, ymax, xe, ye, min(zz + 1, z1), b, be);
+    // Two row buffers: the current plane and the next plane in flight.  The
+    // prefetch is unconditional (the last one re-reads plane z1) so the wait
+    // for one buffer never covers the other's loads.  On the common path a
+    // plane costs 17 compares folded into per-lane column bits and two
+    // ballots; codes are formed only for slices whose planes are mixed.
+    const uint32_t full = (1u << (kBlock + 1)) - 1u;
+    float A[kBlock + 1], B[kBlock + 1], ea = 0.0f, eb_ = 0.0f;
+    ClassifyCarry c{};
+    classify_load_plane(field, roff, eoff, load_e, z0, A, ea);
+    auto step = [&](const float(&cur)[kBlock + 1], float& cure, float(&nxt)[kBlock + 1], float& nxte, int zz,
+                    int parity) {
+        classify_load_plane(field, roff, eoff, load_e, min(zz + 1, z1), nxt, nxte);
         __builtin_amdgcn_sched_barrier(0);
-        classify_plane(w, g, T, a, ae, zz > z0, zz, c, level4, lut);
+        const uint32_t col = classify_column(w, T, cur);
+        uint64_t any = __ballot(col != 0);
+        uint64_t all = __ballot(col == full);
+        const uint32_t first = __builtin_amdgcn_readfirstlane(col);
+        if (lane == 0) xch[parity][wv] = first;
+        __syncthreads();
+        uint64_t eb;
+        if (!last) eb = __builtin_amdgcn_readfirstlane(xch[parity][wv + 1]);
+        else eb = __ballot(w.lane_e && cure < T);
+        any |= eb;
+        // all-inside needs every row's right-neighbour sample inside as well
+        if (!w.has_right || (eb & erows) != erows) all = 0;
+        if (zz > z0) {
+            uint32_t acc = 0;
+            const bool uniform_out = (c.any | any) == 0ull;
+            const bool uniform_in = (c.all & all) == ~0ull;
+            if (!(uniform_out || uniform_in)) acc = classify_slice_counts(w, c.col, col, c.eb, eb, lut);
+            if ((lane & 15) == 0 && w.x < w.cx) level4[block_key(g, w.x, w.y0, zz - 1)] = acc;
+        }
+        c.col = col;
+        c.eb = eb;
+        c.any = any;
+        c.all = all;
+    };
+    for (int zz = z0;;) {
+        step(A, ea, B, eb_, zz, 0);
         if (++zz > z1) break;
-        classify_load_plane(field, xl, w.y0, ymax, xe, ye, min(zz + 1, z1), a, ae);
-        __builtin_amdgcn_sched_barrier(0);
-        classify_plane(w, g, T, b, be, true, zz, c, level4, lut);
+        step(B, eb_, A, ea, zz, 1);
         if (++zz > z1) break;
     }
 }
@@ -672,14 +731,15 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
 // Kernel entry points for one field source.  MAKE_FIELD builds the field
 // object from the Geom `g` and the opaque source pointer `fsrc`.
 #define HPMC_DEFINE_FIELD_KERNELS(SUFFIX, MAKE_FIELD)                                          \
-    extern "C" __global__ __launch_bounds__(256) void hpmc_classify_##SUFFIX(                  \
+    extern "C" __global__ __launch_bounds__(1024) void hpmc_classify_##SUFFIX(                 \
         hpmc::Geom g, float T, const void* fsrc, uint32_t* level4, int zseg,                  \
         const uint64_t* cases)                                                                \
     {                                                                                         \
         __shared__ uint8_t lut[256];                                                          \
-        lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);                              \
+        __shared__ uint32_t xch[2][hpmc::kClassifyMaxWaves];                                   \
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) lut[i] = (uint8_t)(cases[i] & 0xfu); \
         __syncthreads();                                                                      \
-        hpmc::classify_body(MAKE_FIELD, g, T, level4, zseg, lut);                             \
+        hpmc::classify_body(MAKE_FIELD, g, T, level4, zseg, lut, xch);                        \
     }                                                                                         \
     extern "C" __global__ __launch_bounds__(256) void hpmc_emit_##SUFFIX(                      \
         hpmc::Geom g, float T, const void* fsrc, const uint2* active, const hpmc::Totals* totals, \
