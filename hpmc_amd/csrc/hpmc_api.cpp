@@ -30,17 +30,21 @@
 HPMC_DEFINE_FIELD_KERNELS(volume, (hpmc::VolumeField{(const float*)fsrc, g.sx, g.sy, g.sz}))
 
 extern "C" __global__ __launch_bounds__(256) void hpmc_scan_reduce(const uint32_t* level4, uint32_t n,
-                                                                   uint2* tile_sums)
+                                                                   uint32_t ntiles, const uint16_t* colcnt,
+                                                                   uint32_t ncc, uint2* tile_sums)
 {
-    hpmc::scan_reduce_body(level4, n, tile_sums);
+    hpmc::scan_reduce_body(level4, n, ntiles, colcnt, ncc, tile_sums);
 }
 
-extern "C" __global__ __launch_bounds__(256) void hpmc_scan_scatter(const uint32_t* level4, uint32_t n,
-                                                                    const uint2* tile_sums, uint32_t ntiles,
-                                                                    uint32_t* l4_off, uint2* active,
+extern "C" __global__ __launch_bounds__(256) void hpmc_scan_scatter(hpmc::Geom g, const uint32_t* level4,
+                                                                    uint32_t n, uint32_t ntiles,
+                                                                    const uint16_t* colcnt, uint32_t ncc,
+                                                                    const uint2* tile_sums, uint32_t nctiles,
+                                                                    uint32_t* l4_off, uint2* items,
                                                                     hpmc::Totals* totals, hpmc::Totals* host_totals)
 {
-    hpmc::scan_scatter_body(level4, n, tile_sums, ntiles, l4_off, active, totals, host_totals);
+    hpmc::scan_scatter_body(g, level4, n, ntiles, colcnt, ncc, tile_sums, nctiles, l4_off, items, totals,
+                            host_totals);
 }
 
 namespace {
@@ -83,7 +87,11 @@ struct HPMCHistoPyramid {
     uint2* tile_sums = nullptr;
     uint32_t n_tiles = 0;
     uint32_t* l4_off = nullptr;
-    uint2* active = nullptr;
+    uint16_t* colcnt = nullptr;      // block vertex counts in (z segment, y, x, z) order
+    uint32_t n_cc = 0;
+    uint32_t n_ctiles = 0;
+    uint2* items = nullptr;          // emit work items: block key, first vertex | count << 16
+    uint64_t n_items_max = 0;
     int emit_grid = 0;
     hpmc::Totals* d_totals = nullptr;
     hpmc::Totals* h_totals = nullptr;  // pinned
@@ -156,7 +164,8 @@ void free_device(HPMCHistoPyramid* h)
     if (h->level4) (void)hipFree(h->level4);
     if (h->tile_sums) (void)hipFree(h->tile_sums);
     if (h->l4_off) (void)hipFree(h->l4_off);
-    if (h->active) (void)hipFree(h->active);
+    if (h->colcnt) (void)hipFree(h->colcnt);
+    if (h->items) (void)hipFree(h->items);
     if (h->d_totals) (void)hipFree(h->d_totals);
     if (h->h_totals) (void)hipHostFree(h->h_totals);
     if (h->ev_totals) (void)hipEventDestroy(h->ev_totals);
@@ -164,7 +173,8 @@ void free_device(HPMCHistoPyramid* h)
     h->level4 = nullptr;
     h->tile_sums = nullptr;
     h->l4_off = nullptr;
-    h->active = nullptr;
+    h->colcnt = nullptr;
+    h->items = nullptr;
     h->d_totals = nullptr;
     h->h_totals = nullptr;
     h->ev_totals = nullptr;
@@ -319,6 +329,13 @@ int choose_zseg(const hpmc::Geom& g, int resident_wgs)
     return (int)z;
 }
 
+int emit_grid(const HPMCHistoPyramid* h)
+{
+    // persistent; a multiple of 8, never many more waves than work items can be
+    const uint64_t max_wg = ((h->n_items_max + hpmc::kEmitWaves - 1) / hpmc::kEmitWaves + 7) / 8 * 8;
+    return (int)std::min<uint64_t>((uint64_t)h->emit_grid, max_wg);
+}
+
 // hpmc/src/init.cpp:48-69: layout, storage and builder programs, once per
 // configuration.
 bool setup(HPMCHistoPyramid* h)
@@ -378,16 +395,27 @@ bool setup(HPMCHistoPyramid* h)
     h->n_tiles = (uint32_t)((h->n_level4 + hpmc::kScanTile - 1) / hpmc::kScanTile);
     HIP_TRY(h, hipMalloc(&h->level4, h->n_level4 * sizeof(uint32_t)));
     HIP_TRY(h, hipMemset(h->level4, 0, h->n_level4 * sizeof(uint32_t)));
-    HIP_TRY(h, hipMalloc(&h->tile_sums, h->n_tiles * sizeof(uint2)));
+    g.zseg = h->zseg;
+    g.nbx = (g.cx + hpmc::kBlock - 1) / hpmc::kBlock;
+    g.nby = (g.cy + hpmc::kBlock - 1) / hpmc::kBlock;
+    g.npz = (g.cz + h->zseg - 1) / h->zseg;
+    h->n_cc = (uint32_t)((uint64_t)g.nbx * g.nby * g.npz * h->zseg);
+    h->n_ctiles = (h->n_cc + hpmc::kScanTile - 1) / hpmc::kScanTile;
+    // a block holds at most 256 * 15 vertices: 15 work items
+    h->n_items_max = (uint64_t)g.nbx * g.nby * g.cz * 15;
+    HIP_TRY(h, hipMalloc(&h->colcnt, (size_t)h->n_cc * sizeof(uint16_t)));
+    HIP_TRY(h, hipMemset(h->colcnt, 0, (size_t)h->n_cc * sizeof(uint16_t)));
+    HIP_TRY(h, hipMalloc(&h->items, (size_t)h->n_items_max * sizeof(uint2)));
+    HIP_TRY(h, hipMalloc(&h->tile_sums, (h->n_tiles + h->n_ctiles) * sizeof(uint2)));
     HIP_TRY(h, hipMalloc(&h->l4_off, h->n_level4 * sizeof(uint32_t)));
-    HIP_TRY(h, hipMalloc(&h->active, h->n_level4 * sizeof(uint2)));
     {
         // persistent emit grid: as many 4-wave workgroups as fit at once
         int dev = 0, cus = 0, per_cu = 0;
         HIP_TRY(h, hipGetDevice(&dev));
         HIP_TRY(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)hpmc_emit_volume, 256, 0));
-        h->emit_grid = (cus > 0 ? cus : 256) * (per_cu > 0 ? per_cu : 4);
+        // a multiple of 8 for the XCD-aware item mapping
+        h->emit_grid = ((cus > 0 ? cus : 256) * (per_cu > 0 ? per_cu : 4) + 7) / 8 * 8;
     }
     HIP_TRY(h, hipMalloc(&h->d_totals, sizeof(hpmc::Totals)));
     HIP_TRY(h, hipMemset(h->d_totals, 0, sizeof(hpmc::Totals)));
@@ -429,15 +457,16 @@ bool launch_classify(HPMCHistoPyramid* h, float T)
     stage_mark(h, kStageClassify);
     if (h->mode == kFieldVolume) {
         hipLaunchKernelGGL(hpmc_classify_volume, dim3(blocks), dim3(threads), 0, h->stream, g, T, fsrc, h->level4,
-                           h->zseg, (const uint64_t*)h->constants->d_cases);
+                           h->colcnt, h->zseg, (const uint64_t*)h->constants->d_cases);
     } else {
         hpmc::Geom ga = g;
         float Ta = T;
         const void* fa = fsrc;
         uint32_t* l4 = h->level4;
+        uint16_t* aw = h->colcnt;
         int zs = h->zseg;
         const uint64_t* cs = h->constants->d_cases;
-        void* args[] = {&ga, &Ta, &fa, &l4, &zs, &cs};
+        void* args[] = {&ga, &Ta, &fa, &l4, &aw, &zs, &cs};
         HIP_TRY(h, hipModuleLaunchKernel(h->k_classify, blocks, 1, 1, threads, 1, 1, 0, h->stream, args, nullptr));
     }
     HIP_TRY(h, hipGetLastError());
@@ -449,19 +478,15 @@ bool launch_scan(HPMCHistoPyramid* h)
 {
     const uint32_t n = (uint32_t)h->n_level4;
     stage_mark(h, kStageScan);
-    hipLaunchKernelGGL(hpmc_scan_reduce, dim3(h->n_tiles), dim3(256), 0, h->stream, h->level4, n, h->tile_sums);
-    hipLaunchKernelGGL(hpmc_scan_scatter, dim3(h->n_tiles), dim3(256), 0, h->stream, h->level4, n,
-                       (const uint2*)h->tile_sums, h->n_tiles, h->l4_off, h->active, h->d_totals, h->h_totals);
+    const unsigned grid = h->n_tiles + h->n_ctiles;
+    hipLaunchKernelGGL(hpmc_scan_reduce, dim3(grid), dim3(256), 0, h->stream, h->level4, n, h->n_tiles,
+                       (const uint16_t*)h->colcnt, h->n_cc, h->tile_sums);
+    hipLaunchKernelGGL(hpmc_scan_scatter, dim3(grid), dim3(256), 0, h->stream, h->geom, h->level4, n, h->n_tiles,
+                       (const uint16_t*)h->colcnt, h->n_cc, (const uint2*)h->tile_sums, h->n_ctiles, h->l4_off,
+                       h->items, h->d_totals, h->h_totals);
     HIP_TRY(h, hipGetLastError());
     stage_mark(h, kStageScan);
     return true;
-}
-
-int emit_grid(const HPMCHistoPyramid* h)
-{
-    // persistent: never more waves than there are level-4 blocks
-    const uint64_t max_wg = (h->n_level4 + hpmc::kEmitWaves - 1) / hpmc::kEmitWaves;
-    return (int)std::min<uint64_t>((uint64_t)h->emit_grid, max_wg);
 }
 
 bool launch_emit(HPMCHistoPyramid* h, float* out, hpmc::CellEdge* out_ce, uint32_t capacity)
@@ -472,19 +497,20 @@ bool launch_emit(HPMCHistoPyramid* h, float* out, hpmc::CellEdge* out_ce, uint32
     stage_mark(h, kStageEmit);
     if (h->mode == kFieldVolume) {
         hipLaunchKernelGGL(hpmc_emit_volume, dim3(grid), dim3(256), 0, h->stream, g, h->threshold, fsrc,
-                           (const uint2*)h->active, (const hpmc::Totals*)h->d_totals, out, out_ce, capacity,
-                           (const uint64_t*)h->constants->d_cases);
+                           (const uint2*)h->items, (const uint32_t*)h->l4_off, (const hpmc::Totals*)h->d_totals,
+                           out, out_ce, capacity, (const uint64_t*)h->constants->d_cases);
     } else {
         hpmc::Geom ga = g;
         float Ta = h->threshold;
         const void* fa = fsrc;
-        const uint2* ac = h->active;
+        const uint2* li = h->items;
+        const uint32_t* lo = h->l4_off;
         const hpmc::Totals* tt = h->d_totals;
         float* o = out;
         hpmc::CellEdge* oc = out_ce;
         uint32_t cap = capacity;
         const uint64_t* cs = h->constants->d_cases;
-        void* args[] = {&ga, &Ta, &fa, &ac, &tt, &o, &oc, &cap, &cs};
+        void* args[] = {&ga, &Ta, &fa, &li, &lo, &tt, &o, &oc, &cap, &cs};
         HIP_TRY(h, hipModuleLaunchKernel(h->k_emit, grid, 1, 1, 256, 1, 1, 0, h->stream, args, nullptr));
     }
     HIP_TRY(h, hipGetLastError());

@@ -48,12 +48,14 @@ struct Geom {
     int blocks_log2;      // log2 of the level-4 side (l2 + 1 - kBlockLog2)
     float scx, scy, scz;  // extent / cells
     float ipx, ipy, ipz;  // 1 / lattice size (snippet texel-centre coordinates)
+    int zseg;             // classification slices per workgroup
+    int nbx, nby, npz;    // level-4 block columns along x, y; z segments
 };
 
 struct Totals {
     uint32_t vertices;    // N
     uint32_t blocks;      // non-empty level-4 blocks
-    uint32_t overflow;    // vertices dropped by an extraction (capacity)
+    uint32_t items;       // emit work items
     uint32_t pad;
 };
 
@@ -248,8 +250,9 @@ __device__ __forceinline__ void classify_tile(const Geom& g, int zseg, int wgx, 
 
 template <class F>
 __device__ __forceinline__ void classify_body(const F& field, const Geom& g, float T,
-                                              uint32_t* __restrict__ level4, int zseg,
-                                              const uint8_t* __restrict__ lut, uint32_t (*xch)[kClassifyMaxWaves])
+                                              uint32_t* __restrict__ level4, uint16_t* __restrict__ colcnt,
+                                              int zseg, const uint8_t* __restrict__ lut,
+                                              uint32_t (*xch)[kClassifyMaxWaves])
 {
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -292,6 +295,9 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
     // plane costs 17 compares folded into per-lane column bits and two
     // ballots; codes are formed only for slices whose planes are mixed.
     const uint32_t full = (1u << (kBlock + 1)) - 1u;
+    // this lane's block column in the column-major count array (lanes 0, 16,
+    // 32, 48 own one block column each)
+    uint16_t* ccol = colcnt + (((size_t)pz * g.nby + py) * g.nbx + (w.x >> kBlockLog2)) * g.zseg;
     float A[kBlock + 1], B[kBlock + 1], ea = 0.0f, eb_ = 0.0f;
     ClassifyCarry c{};
     classify_load_plane(field, roff, eoff, load_e, z0, A, ea);
@@ -317,6 +323,7 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
             const bool uniform_in = (c.all & all) == ~0ull;
             if (!(uniform_out || uniform_in)) acc = classify_slice_counts(w, c.col, col, c.eb, eb, lut);
             if ((lane & 15) == 0 && w.x < w.cx) level4[block_key(g, w.x, w.y0, zz - 1)] = acc;
+            if ((lane & 15) == 0 && w.x < w.cx) ccol[zz - 1 - z0] = (uint16_t)acc;
         }
         c.col = col;
         c.eb = eb;
@@ -377,68 +384,121 @@ __device__ __forceinline__ void scan_load16(const uint32_t* __restrict__ level4,
     }
 }
 
-__device__ __forceinline__ void scan_reduce_body(const uint32_t* __restrict__ level4, uint32_t n,
+// Emit work items hold at most kEmitChunk vertices of one block, so a busy
+// block is spread over several waves.
+constexpr int kEmitChunk = 256;
+
+__device__ __forceinline__ uint32_t chunks_of(uint32_t cnt) { return (cnt + kEmitChunk - 1) / kEmitChunk; }
+
+// The 16 column-major block counts of thread t of a tile (u16, two 16-byte
+// loads); entries past n read as zero.
+__device__ __forceinline__ void scan_load16_u16(const uint16_t* __restrict__ cc, uint32_t n, uint32_t base,
+                                                uint32_t (&c)[kScanPer])
+{
+    if (base + kScanPer <= n) {
+        const uint4* p = (const uint4*)(cc + base);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint4 q = p[k];
+            const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                c[8 * k + 2 * i] = w[i] & 0xffffu;
+                c[8 * k + 2 * i + 1] = w[i] >> 16;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) c[k] = (base + k < n) ? cc[base + k] : 0u;
+    }
+}
+
+// Tiles [0, ntiles) cover the level-4 array (Morton order): sums of counts
+// (x) and of non-empty blocks (y).  Tiles [ntiles, ntiles + nctiles) cover
+// the column-major counts: vertices (x) and emit work items (y).
+__device__ __forceinline__ void scan_reduce_body(const uint32_t* __restrict__ level4, uint32_t n, uint32_t ntiles,
+                                                 const uint16_t* __restrict__ colcnt, uint32_t ncc,
                                                  uint2* __restrict__ tile_sums)
 {
     __shared__ uint2 sh[16];
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
-    uint32_t c[kScanPer];
-    scan_load16(level4, n, base, c);
     uint2 s = make_uint2(0, 0);
+    uint32_t c[kScanPer];
+    if (blockIdx.x < ntiles) {
+        scan_load16(level4, n, blockIdx.x * kScanTile + threadIdx.x * kScanPer, c);
 #pragma unroll
-    for (int k = 0; k < kScanPer; ++k) {
-        s.x += c[k];
-        s.y += c[k] != 0;
+        for (int k = 0; k < kScanPer; ++k) {
+            s.x += c[k];
+            s.y += c[k] != 0;
+        }
+    } else {
+        scan_load16_u16(colcnt, ncc, (blockIdx.x - ntiles) * kScanTile + threadIdx.x * kScanPer, c);
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            s.x += c[k];
+            s.y += chunks_of(c[k]);
+        }
     }
     s = block_reduce_u2(s, sh);
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = s;
 }
 
-// scan, part 2: every tile re-reads the preceding tile sums (a few hundred),
-// scans its own 4096 entries and writes, for each non-empty level-4 block,
-// the exclusive vertex offset of its first vertex in Morton order (the
-// reference's key order).  Tile 0 also writes the totals, to device memory
-// and to the host-visible copy.
-__device__ __forceinline__ void scan_scatter_body(const uint32_t* __restrict__ level4, uint32_t n,
-                                                  const uint2* __restrict__ tile_sums, uint32_t ntiles,
-                                                  uint32_t* __restrict__ l4_off, uint2* __restrict__ active,
+// Level-4 tiles: every tile re-reads the preceding tile sums (a few
+// hundred), scans its own 4096 entries and writes, for each non-empty
+// block, the exclusive vertex offset of its first vertex in Morton order
+// (the reference's key order).  Tile 0 also writes the totals, to device
+// memory and to the host-visible copy.
+// Column tiles: the same scan over the column-major counts, writing the
+// emit work items -- (block key, first vertex | count << 16), one per
+// kEmitChunk vertices of a block -- in (z segment, y, x, z) order, so that
+// consecutive items are consecutive slices of one block column.
+__device__ __forceinline__ void scan_scatter_body(const Geom& g, const uint32_t* __restrict__ level4, uint32_t n,
+                                                  uint32_t ntiles, const uint16_t* __restrict__ colcnt, uint32_t ncc,
+                                                  const uint2* __restrict__ tile_sums, uint32_t nctiles,
+                                                  uint32_t* __restrict__ l4_off, uint2* __restrict__ items,
                                                   Totals* __restrict__ totals, Totals* __restrict__ host_totals)
 {
     __shared__ uint2 sh[16];
     __shared__ uint2 wsum[16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+    const bool is_col = blockIdx.x >= ntiles;
+    const uint32_t t0 = is_col ? ntiles : 0u;
+    const uint32_t tile = blockIdx.x - t0;
+    const uint32_t base = tile * kScanTile + threadIdx.x * kScanPer;
     uint32_t c[kScanPer];
-    scan_load16(level4, n, base, c);
+    uint2 mine = make_uint2(0, 0);
+    if (!is_col) {
+        scan_load16(level4, n, base, c);
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) mine.x += c[k];
+    } else {
+        scan_load16_u16(colcnt, ncc, base, c);
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) mine.y += chunks_of(c[k]);
+    }
     uint2 pre = make_uint2(0, 0);
-    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += blockDim.x) {
-        pre.x += tile_sums[i].x;
-        pre.y += tile_sums[i].y;
+    for (uint32_t i = threadIdx.x; i < tile; i += blockDim.x) {
+        pre.x += tile_sums[t0 + i].x;
+        pre.y += tile_sums[t0 + i].y;
     }
     pre = block_reduce_u2(pre, sh);
     if (blockIdx.x == 0) {
-        uint2 tot = make_uint2(0, 0);
+        uint2 tot = make_uint2(0, 0), items_tot = make_uint2(0, 0);
         for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) {
             tot.x += tile_sums[i].x;
             tot.y += tile_sums[i].y;
         }
+        for (uint32_t i = threadIdx.x; i < nctiles; i += blockDim.x) items_tot.y += tile_sums[ntiles + i].y;
         tot = block_reduce_u2(tot, sh);
+        items_tot = block_reduce_u2(items_tot, sh);
         if (threadIdx.x == 0) {
             Totals t;
             t.vertices = tot.x;
             t.blocks = tot.y;
-            t.overflow = 0;
+            t.items = items_tot.y;
             t.pad = 0;
             *totals = t;
             if (host_totals) *host_totals = t;
         }
-    }
-    // vertex offsets (x) and non-empty block ranks (y), exclusive
-    uint2 mine = make_uint2(0, 0);
-#pragma unroll
-    for (int k = 0; k < kScanPer; ++k) {
-        mine.x += c[k];
-        mine.y += c[k] != 0;
     }
     uint2 inc = mine;
 #pragma unroll
@@ -456,36 +516,51 @@ __device__ __forceinline__ void scan_scatter_body(const uint32_t* __restrict__ l
         off.x += wsum[i].x;
         off.y += wsum[i].y;
     }
+    if (!is_col) {
 #pragma unroll
-    for (int k = 0; k < kScanPer; ++k) {
-        if (c[k]) {
-            l4_off[base + k] = off.x;
-            active[off.y] = make_uint2(base + k, off.x);
+        for (int k = 0; k < kScanPer; ++k) {
+            if (c[k]) l4_off[base + k] = off.x;
             off.x += c[k];
-            off.y += 1;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            if (!c[k]) continue;
+            const uint32_t ci = base + k;
+            const uint32_t wi = ci / (uint32_t)g.zseg;
+            const int zi = (int)(ci - wi * (uint32_t)g.zseg);
+            const int bxb = (int)(wi % (uint32_t)g.nbx);
+            const uint32_t rest = wi / (uint32_t)g.nbx;
+            const int by = (int)(rest % (uint32_t)g.nby);
+            const int pz = (int)(rest / (uint32_t)g.nby);
+            const uint32_t key = block_key(g, bxb << kBlockLog2, by << kBlockLog2, pz * g.zseg + zi);
+            for (uint32_t v = 0; v < c[k]; v += kEmitChunk) items[off.y++] = make_uint2(key, v | (c[k] << 16));
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// emit: a persistent grid; each wave takes non-empty level-4 blocks from the
-// compact list the scan wrote (Morton order, so consecutive items are
-// neighbours in the base image), one block per step.
+// emit: a persistent grid over the work items of the column scan; each item
+// is up to kEmitChunk vertices of one non-empty level-4 block.  The four
+// waves of a workgroup take four consecutive items and a workgroup's
+// successor on the same XCD the next four, so blocks sharing lattice planes
+// (consecutive slices of a column) are processed together in one L2.
 //
 // A block needs lattice planes z, z+1 (corners) and z+2 (forward
-// differences) over 18x18 samples.  They are loaded into registers one item
-// ahead (the key two items ahead) and staged in the wave's LDS region.  For
-// a block the 64 lanes classify its 256 cells (a 2x2 quad each, Morton
+// differences) over 18x18 samples, staged in the wave's LDS; the next
+// item's planes and output offset are loaded while the current item runs.
+// For a block the 64 lanes classify its 256 cells (a 2x2 quad each, Morton
 // order), build the block's pyramid levels 0..3 in LDS (cells, quads, 16-
 // and 64-cell groups: four children each, ordered (0,0), (1,0), (0,1), (1,1)
 // as in the reference's traversal, shadergen.cpp:448-495), and then each
 // lane takes output vertices, descends the four levels to its cell and
-// writes the interpolated position and normal.
+// writes the interpolated position and normal at the block's offset in the
+// reference's vertex order.
 constexpr int kEmitWaves = 4;
 constexpr int kPlaneLoads = (kBrickPlane + 63) / 64;   // 6 per lane per plane
 
 struct EmitWaveShared {
-    float plane[3][kBrickPlane];   // planes z, z+1, z+2 of the current block
+    float plane[3][kBrickPlane];   // ring: plane p in slot p % 3
     uint16_t l0[256];              // per-cell vertex counts (Morton order)
     uint16_t l1[64];               // per-quad sums
     uint16_t l2[16];
@@ -508,26 +583,18 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One level of the in-block descent: pick the child whose range holds k.
-__device__ __forceinline__ int descend4(const uint16_t* sums, uint32_t& k)
+// One level of the in-block descent.  A node holds the inclusive prefix
+// sums of its four children (p0 <= p1 <= p2 <= p3); pick the child whose
+// range holds k and make k relative to it.  Branch-free.
+__device__ __forceinline__ int descend4(const uint16_t* node, uint32_t& k)
 {
-    const uint64_t q = *(const uint64_t*)sums;
-    const uint32_t s0 = (uint32_t)(q & 0xffff), s1 = (uint32_t)((q >> 16) & 0xffff);
-    const uint32_t s2 = (uint32_t)((q >> 32) & 0xffff);
-    int j = 0;
-    if (k >= s0) {
-        k -= s0;
-        j = 1;
-        if (k >= s1) {
-            k -= s1;
-            j = 2;
-            if (k >= s2) {
-                k -= s2;
-                j = 3;
-            }
-        }
-    }
-    return j;
+    const uint64_t q = *(const uint64_t*)node;
+    const uint32_t p0 = (uint32_t)(q & 0xffff), p1 = (uint32_t)((q >> 16) & 0xffff);
+    const uint32_t p2 = (uint32_t)((q >> 32) & 0xffff);
+    const bool g0 = k >= p0, g1 = k >= p1, g2 = k >= p2;
+    const uint32_t sub = g2 ? p2 : (g1 ? p1 : (g0 ? p0 : 0u));
+    k -= sub;
+    return (int)g0 + (int)g1 + (int)g2;
 }
 
 // Block origin (x0, y0, z) of a level-4 key.
@@ -541,12 +608,11 @@ __device__ __forceinline__ void block_origin(const Geom& g, uint32_t key, int& x
     y0 = Y0 - ty * g.tsy2;
 }
 
+// Loads planes p_lo..p_hi (1 to 3) of the 18x18 window at (x0, y0) into r.
 template <class F>
-__device__ __forceinline__ void emit_fetch_block(const F& field, const Geom& g, uint32_t key, int lane,
-                                                 float (&r)[3][kPlaneLoads])
+__device__ __forceinline__ void emit_fetch(const F& field, int x0, int y0, int p_lo, int p_hi, int lane,
+                                           float (&r)[3][kPlaneLoads])
 {
-    int x0, y0, z;
-    block_origin(g, key, x0, y0, z);
     uint32_t idx[kPlaneLoads];
 #pragma unroll
     for (int k = 0; k < kPlaneLoads; ++k) {
@@ -556,24 +622,47 @@ __device__ __forceinline__ void emit_fetch_block(const F& field, const Geom& g, 
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        const typename F::Plane pl = field.plane(z + q);
+        if (p_lo + q <= p_hi) {
+            const typename F::Plane pl = field.plane(p_lo + q);
 #pragma unroll
-        for (int k = 0; k < kPlaneLoads; ++k) r[q][k] = field.fetch(pl, idx[k]);
+            for (int k = 0; k < kPlaneLoads; ++k) r[q][k] = field.fetch(pl, idx[k]);
+        }
+    }
+}
+
+__device__ __forceinline__ void emit_store(EmitWaveShared& ws, int p_lo, int p_hi, int lane,
+                                           const float (&r)[3][kPlaneLoads])
+{
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (p_lo + q <= p_hi) {
+            float* dst = ws.plane[(p_lo + q) % 3];
+#pragma unroll
+            for (int k = 0; k < kPlaneLoads; ++k) {
+                const int i = lane + 64 * k;
+                if (i < kBrickPlane) dst[i] = r[q][k];
+            }
+        }
     }
 }
 
 template <class F>
 __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T,
-                                          const uint2* __restrict__ active, const Totals* __restrict__ totals,
+                                          const uint2* __restrict__ items, const uint32_t* __restrict__ l4_off,
+                                          const Totals* __restrict__ totals,
                                           float* __restrict__ out, CellEdge* __restrict__ out_ce,
                                           uint32_t capacity, EmitShared& sh)
 {
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const uint32_t nblocks = __builtin_amdgcn_readfirstlane(totals->blocks);
-    const uint32_t stride = gridDim.x * kEmitWaves;
-    uint32_t item = __builtin_amdgcn_readfirstlane(blockIdx.x * kEmitWaves + wv);
-    if (item >= nblocks) return;
+    const uint32_t nitems = __builtin_amdgcn_readfirstlane(totals->items);
+    const uint32_t nwg = gridDim.x;                 // a multiple of 8
+    const uint32_t stride = nwg * kEmitWaves;
+    // workgroups are dealt to the 8 XCDs round robin; give each XCD a
+    // contiguous run of logical workgroup indices
+    const uint32_t t = (blockIdx.x & 7u) * (nwg >> 3) + (blockIdx.x >> 3);
+    uint32_t it = __builtin_amdgcn_readfirstlane(t * kEmitWaves + wv);
+    if (it >= nitems) return;
 
     EmitWaveShared& ws = sh.w[wv];
     const uint64_t* cases = sh.cases;
@@ -582,43 +671,41 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
     const int qx = (int)morton_compact((uint32_t)lane) * 2;
     const int qy = (int)morton_compact((uint32_t)lane >> 1) * 2;
 
-    uint2 cur = active[item];
+    uint2 cur = items[it];
     cur.x = __builtin_amdgcn_readfirstlane(cur.x);
     cur.y = __builtin_amdgcn_readfirstlane(cur.y);
+    uint32_t obase = l4_off[cur.x];
     float r[3][kPlaneLoads];
-    emit_fetch_block(field, g, cur.x, lane, r);
-    uint32_t next_item = item + stride;
-    uint2 nxt = make_uint2(0, 0);
-    if (next_item < nblocks) nxt = active[next_item];
+    int x0, y0, z;
+    block_origin(g, cur.x, x0, y0, z);
+    emit_fetch(field, x0, y0, z, z + 2, lane, r);
+    uint2 nxt = items[min(it + stride, nitems - 1)];
 
     for (;;) {
-        const uint32_t key = cur.x, base = cur.y;
-        int x0, y0, z;
-        block_origin(g, key, x0, y0, z);
+        const uint32_t vlo = cur.y & 0xffffu, cnt = cur.y >> 16;
+        const uint32_t base = __builtin_amdgcn_readfirstlane(obase);
+        const int cz = z, cx0 = x0, cy0 = y0;
         wave_lds_sync();
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-#pragma unroll
-            for (int k = 0; k < kPlaneLoads; ++k) {
-                const int i = lane + 64 * k;
-                if (i < kBrickPlane) ws.plane[q][i] = r[q][k];
-            }
-        // prefetch the next item's planes and the key after it
-        const bool more = next_item < nblocks;
-        uint2 after = make_uint2(0, 0);
+        emit_store(ws, 0, 2, lane, r);
+        wave_lds_sync();
+
+        // next item: entry, output offset and planes in flight meanwhile
+        const bool more = it + stride < nitems;
+        uint2 after = nxt;
         if (more) {
             nxt.x = __builtin_amdgcn_readfirstlane(nxt.x);
             nxt.y = __builtin_amdgcn_readfirstlane(nxt.y);
-            emit_fetch_block(field, g, nxt.x, lane, r);
-            if (next_item + stride < nblocks) after = active[next_item + stride];
+            obase = l4_off[nxt.x];
+            block_origin(g, nxt.x, x0, y0, z);
+            emit_fetch(field, x0, y0, z, z + 2, lane, r);
+            after = items[min(it + 2 * stride, nitems - 1)];
         }
-        wave_lds_sync();
 
         const float* P0 = ws.plane[0];
         const float* P1 = ws.plane[1];
         const float* P2 = ws.plane[2];
-        const bool qvx0 = x0 + qx < g.cx, qvx1 = x0 + qx + 1 < g.cx;
-        const bool qvy0 = y0 + qy < g.cy, qvy1 = y0 + qy + 1 < g.cy;
+        const bool qvx0 = cx0 + qx < g.cx, qvx1 = cx0 + qx + 1 < g.cx;
+        const bool qvy0 = cy0 + qy < g.cy, qvy1 = cy0 + qy + 1 < g.cy;
 
         // classify the quad: corners (qx..qx+2, qy..qy+2) of planes z, z+1
         uint32_t bits0 = 0, bits1 = 0;   // 3x3 inside bits per plane, bit = dx + 3*dy
@@ -636,32 +723,37 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
         for (int q = 0; q < 4; ++q) {
             const int cxq = q & 1, cyq = q >> 1;
             const int sh0 = cxq + 3 * cyq;
-            const uint32_t lo = ((bits0 >> sh0) & 3u) | (((bits0 >> (sh0 + 3)) & 3u) << 2);
-            const uint32_t hi = ((bits1 >> sh0) & 3u) | (((bits1 >> (sh0 + 3)) & 3u) << 2);
-            const uint32_t code = lo | (hi << 4);
+            const uint32_t lo2 = ((bits0 >> sh0) & 3u) | (((bits0 >> (sh0 + 3)) & 3u) << 2);
+            const uint32_t hi2 = ((bits1 >> sh0) & 3u) | (((bits1 >> (sh0 + 3)) & 3u) << 2);
+            const uint32_t code = lo2 | (hi2 << 4);
             const bool valid = (cxq ? qvx1 : qvx0) && (cyq ? qvy1 : qvy0);
             const uint32_t c = lut[code];
             const uint32_t cv = valid ? c : 0u;
             codes4 |= code << (8 * q);
             cnt4 += cv;
-            l0q |= (uint64_t)cv << (16 * q);
+            l0q |= (uint64_t)cnt4 << (16 * q);   // level 0 node: inclusive prefixes
         }
-        // pyramid levels 1..3 of the block
-        uint32_t g4 = cnt4 + __shfl_xor(cnt4, 1);
-        g4 += __shfl_xor(g4, 2);
-        uint32_t g16 = g4 + __shfl_xor(g4, 4);
-        g16 += __shfl_xor(g16, 8);
-        uint32_t g64 = g16 + __shfl_xor(g16, 16);
-        g64 += __shfl_xor(g64, 32);
+        // pyramid levels 1..3: every node stores its children's inclusive
+        // prefix sums, taken from one inclusive scan of the quad counts
+        uint32_t inc = cnt4;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t tt = __shfl_up(inc, o);
+            if (lane >= o) inc += tt;
+        }
+        const uint32_t exc = inc - cnt4;
+        const uint32_t base4 = __shfl(exc, lane & ~3);
+        const uint32_t base16 = __shfl(exc, lane & ~15);
         *(uint64_t*)&ws.l0[4 * lane] = l0q;
         *(uint32_t*)&ws.code[4 * lane] = codes4;
-        ws.l1[lane] = (uint16_t)cnt4;
-        if ((lane & 3) == 0) ws.l2[lane >> 2] = (uint16_t)g4;
-        if ((lane & 15) == 0) ws.l3[lane >> 4] = (uint16_t)g16;
-        const uint32_t nv = g64;   // whole-block total (every lane)
+        ws.l1[lane] = (uint16_t)(inc - base4);
+        if ((lane & 3) == 3) ws.l2[lane >> 2] = (uint16_t)(inc - base16);
+        if ((lane & 15) == 15) ws.l3[lane >> 4] = (uint16_t)inc;
+        const uint32_t nv = min(__builtin_amdgcn_readlane(inc, 63), cnt);   // whole-block total
+        const uint32_t vhi = min(vlo + (uint32_t)kEmitChunk, nv);
         wave_lds_sync();
 
-        for (uint32_t v = lane; v < nv; v += 64) {
+        for (uint32_t v = vlo + lane; v < vhi; v += 64) {
             uint32_t k = v;
             int c = descend4(ws.l3, k);
             c = 4 * c + descend4(ws.l2 + 4 * c, k);
@@ -685,27 +777,29 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
             const float va = Pa[ia], vb = Pb[ib];
             const float nax = Pa[ia + 1], nay = Pa[ia + kBrick], naz = Pan[ia];
             const float nbx_ = Pb[ib + 1], nby_ = Pb[ib + kBrick], nbz_ = Pbn[ib];
-            const float t = (va - T) / (va - vb);
-            const float u = 1.0f - t;
-            const float gx = (float)(x0 + ax), gy = (float)(y0 + ay), gz = (float)(z + az);
+            // va and vb lie on opposite sides of T, so va - vb != 0; the
+            // hardware reciprocal (1 ulp) keeps t within a few ulp
+            const float tt = (va - T) * __builtin_amdgcn_rcpf(va - vb);
+            const float u = 1.0f - tt;
+            const float gx = (float)(cx0 + ax), gy = (float)(cy0 + ay), gz = (float)(cz + az);
             const uint32_t o = base + v;
             if (o < capacity) {
                 float2* dst = (float2*)(out + (size_t)o * 6);
-                dst[0] = make_float2((T - (nax * u + nbx_ * t)) * g.scx, (T - (nay * u + nby_ * t)) * g.scy);
-                dst[1] = make_float2((T - (naz * u + nbz_ * t)) * g.scz, (gx + (axis == 0 ? t : 0.0f)) * g.scx);
-                dst[2] = make_float2((gy + (axis == 1 ? t : 0.0f)) * g.scy, (gz + (axis == 2 ? t : 0.0f)) * g.scz);
+                dst[0] = make_float2((T - (nax * u + nbx_ * tt)) * g.scx, (T - (nay * u + nby_ * tt)) * g.scy);
+                dst[1] = make_float2((T - (naz * u + nbz_ * tt)) * g.scz, (gx + (axis == 0 ? tt : 0.0f)) * g.scx);
+                dst[2] = make_float2((gy + (axis == 1 ? tt : 0.0f)) * g.scy, (gz + (axis == 2 ? tt : 0.0f)) * g.scz);
                 if (out_ce) {
                     CellEdge ce;
-                    ce.cell = (uint32_t)(x0 + ccx) + (uint32_t)g.cx * ((uint32_t)(y0 + ccy) + (uint32_t)g.cy * (uint32_t)z);
+                    ce.cell = (uint32_t)(cx0 + ccx) + (uint32_t)g.cx * ((uint32_t)(cy0 + ccy) + (uint32_t)g.cy * (uint32_t)cz);
                     ce.edge = e;
                     out_ce[o] = ce;
                 }
             }
         }
         if (!more) break;
+        it += stride;
         cur = nxt;
         nxt = after;
-        next_item += stride;
     }
 }
 
@@ -732,24 +826,25 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
 // object from the Geom `g` and the opaque source pointer `fsrc`.
 #define HPMC_DEFINE_FIELD_KERNELS(SUFFIX, MAKE_FIELD)                                          \
     extern "C" __global__ __launch_bounds__(1024) void hpmc_classify_##SUFFIX(                 \
-        hpmc::Geom g, float T, const void* fsrc, uint32_t* level4, int zseg,                  \
-        const uint64_t* cases)                                                                \
+        hpmc::Geom g, float T, const void* fsrc, uint32_t* level4, uint16_t* colcnt,          \
+        int zseg, const uint64_t* cases)                                                      \
     {                                                                                         \
         __shared__ uint8_t lut[256];                                                          \
         __shared__ uint32_t xch[2][hpmc::kClassifyMaxWaves];                                   \
         for (int i = threadIdx.x; i < 256; i += blockDim.x) lut[i] = (uint8_t)(cases[i] & 0xfu); \
         __syncthreads();                                                                      \
-        hpmc::classify_body(MAKE_FIELD, g, T, level4, zseg, lut, xch);                        \
+        hpmc::classify_body(MAKE_FIELD, g, T, level4, colcnt, zseg, lut, xch);                \
     }                                                                                         \
     extern "C" __global__ __launch_bounds__(256) void hpmc_emit_##SUFFIX(                      \
-        hpmc::Geom g, float T, const void* fsrc, const uint2* active, const hpmc::Totals* totals, \
-        float* out, hpmc::CellEdge* out_ce, uint32_t capacity, const uint64_t* cases)         \
+        hpmc::Geom g, float T, const void* fsrc, const uint2* items, const uint32_t* l4_off,   \
+        const hpmc::Totals* totals, float* out, hpmc::CellEdge* out_ce, uint32_t capacity,    \
+        const uint64_t* cases)                                                                \
     {                                                                                         \
         __shared__ hpmc::EmitShared sh;                                                       \
         sh.cases[threadIdx.x] = cases[threadIdx.x];                                           \
         sh.lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);                           \
         __syncthreads();                                                                      \
-        hpmc::emit_body(MAKE_FIELD, g, T, active, totals, out, out_ce, capacity, sh);         \
+        hpmc::emit_body(MAKE_FIELD, g, T, items, l4_off, totals, out, out_ce, capacity, sh);  \
     }                                                                                         \
     extern "C" __global__ __launch_bounds__(256) void hpmc_codes_##SUFFIX(                     \
         hpmc::Geom g, float T, const void* fsrc, uint8_t* codes)                              \
