@@ -29,22 +29,26 @@
 
 HPMC_DEFINE_FIELD_KERNELS(volume, (hpmc::VolumeField{(const float*)fsrc, g.sx, g.sy, g.sz}))
 
-extern "C" __global__ __launch_bounds__(256) void hpmc_scan_reduce(const uint32_t* level4, uint32_t n,
-                                                                   uint32_t ntiles, const uint16_t* colcnt,
-                                                                   uint32_t ncc, uint2* tile_sums)
+extern "C" __global__ __launch_bounds__(256) void hpmc_scan_reduce(hpmc::Geom g, uint32_t n, uint32_t ntiles,
+                                                                   const uint16_t* colcnt, uint32_t ncc,
+                                                                   uint2* tile_sums)
 {
-    hpmc::scan_reduce_body(level4, n, ntiles, colcnt, ncc, tile_sums);
+    hpmc::scan_reduce_body(g, n, ntiles, colcnt, ncc, tile_sums);
 }
 
-extern "C" __global__ __launch_bounds__(256) void hpmc_scan_scatter(hpmc::Geom g, const uint32_t* level4,
-                                                                    uint32_t n, uint32_t ntiles,
+extern "C" __global__ __launch_bounds__(256) void hpmc_scan_scatter(hpmc::Geom g, uint32_t n, uint32_t ntiles,
                                                                     const uint16_t* colcnt, uint32_t ncc,
                                                                     const uint2* tile_sums, uint32_t nctiles,
                                                                     uint32_t* l4_off, uint2* items,
                                                                     hpmc::Totals* totals, hpmc::Totals* host_totals)
 {
-    hpmc::scan_scatter_body(g, level4, n, ntiles, colcnt, ncc, tile_sums, nctiles, l4_off, items, totals,
-                            host_totals);
+    hpmc::scan_scatter_body(g, n, ntiles, colcnt, ncc, tile_sums, nctiles, l4_off, items, totals, host_totals);
+}
+
+extern "C" __global__ __launch_bounds__(256) void hpmc_level4_gather(hpmc::Geom g, const uint16_t* colcnt,
+                                                                     uint32_t n, uint32_t* out)
+{
+    hpmc::level4_body(g, colcnt, n, out);
 }
 
 namespace {
@@ -82,8 +86,7 @@ struct HPMCHistoPyramid {
     hpmc::Geom geom{};
     int zseg = 16;
     // device state
-    uint32_t* level4 = nullptr;
-    uint64_t n_level4 = 0;
+    uint64_t n_level4 = 0;           // level-4 blocks in Morton key order (incl. blocks outside the grid)
     uint2* tile_sums = nullptr;
     uint32_t n_tiles = 0;
     uint32_t* l4_off = nullptr;
@@ -161,7 +164,6 @@ void ok(HPMCHistoPyramid* h)
 
 void free_device(HPMCHistoPyramid* h)
 {
-    if (h->level4) (void)hipFree(h->level4);
     if (h->tile_sums) (void)hipFree(h->tile_sums);
     if (h->l4_off) (void)hipFree(h->l4_off);
     if (h->colcnt) (void)hipFree(h->colcnt);
@@ -170,7 +172,6 @@ void free_device(HPMCHistoPyramid* h)
     if (h->h_totals) (void)hipHostFree(h->h_totals);
     if (h->ev_totals) (void)hipEventDestroy(h->ev_totals);
     if (h->module) (void)hipModuleUnload(h->module);
-    h->level4 = nullptr;
     h->tile_sums = nullptr;
     h->l4_off = nullptr;
     h->colcnt = nullptr;
@@ -248,6 +249,11 @@ std::string rtc_source(const std::string& snippet)
          "        return HPMC_fetch(make_float3(((float)(idx & 0xffffu) + 0.5f) * ipx,\n"
          "                                      ((float)(idx >> 16) + 0.5f) * ipy, pz));\n"
          "    }\n"
+         "    __device__ __forceinline__ uint32_t col_index(int i) const { return (uint32_t)i; }\n"
+         "    __device__ __forceinline__ uint32_t row_index(int j) const { return (uint32_t)j; }\n"
+         "    __device__ __forceinline__ float fetch2(Plane pz, uint32_t col, uint32_t row) const {\n"
+         "        return HPMC_fetch(make_float3(((float)col + 0.5f) * ipx, ((float)row + 0.5f) * ipy, pz));\n"
+         "    }\n"
          "};\n"
          "}\n"
          "HPMC_DEFINE_FIELD_KERNELS(rtc, (hpmc::SnippetField{g.ipx, g.ipy, g.ipz}))\n";
@@ -309,7 +315,8 @@ bool compile_snippet(HPMCHistoPyramid* h)
 // Classification launch shape: one workgroup of ceil(X/64) waves (at most
 // 16) per band of 16 cell rows and zseg slices.  zseg is picked so the
 // workgroups fill the device about once (each extra segment re-reads one
-// plane), between 8 and 64 slices.
+// plane): the power of two in 8..32 nearest above that, 32 being the slots
+// of a block column in the count array.
 int classify_waves(const hpmc::Geom& g)
 {
     int nw = (g.sx + 63) / 64;
@@ -324,9 +331,9 @@ int choose_zseg(const hpmc::Geom& g, int resident_wgs)
     const int64_t bands = wgx * ((g.cy + hpmc::kBlock - 1) / hpmc::kBlock);
     int64_t target = resident_wgs > 0 ? resident_wgs : 1024;
     int64_t z = (g.cz * bands + target - 1) / target;
-    if (z < 8) z = 8;
-    if (z > 64) z = 64;
-    return (int)z;
+    int zs = 8;
+    while (zs < z && zs < hpmc::kColStride) zs *= 2;
+    return zs;
 }
 
 int emit_grid(const HPMCHistoPyramid* h)
@@ -393,13 +400,11 @@ bool setup(HPMCHistoPyramid* h)
     const uint64_t side = (uint64_t)1 << g.blocks_log2;
     h->n_level4 = side * side;
     h->n_tiles = (uint32_t)((h->n_level4 + hpmc::kScanTile - 1) / hpmc::kScanTile);
-    HIP_TRY(h, hipMalloc(&h->level4, h->n_level4 * sizeof(uint32_t)));
-    HIP_TRY(h, hipMemset(h->level4, 0, h->n_level4 * sizeof(uint32_t)));
     g.zseg = h->zseg;
     g.nbx = (g.cx + hpmc::kBlock - 1) / hpmc::kBlock;
     g.nby = (g.cy + hpmc::kBlock - 1) / hpmc::kBlock;
     g.npz = (g.cz + h->zseg - 1) / h->zseg;
-    h->n_cc = (uint32_t)((uint64_t)g.nbx * g.nby * g.npz * h->zseg);
+    h->n_cc = (uint32_t)((uint64_t)g.nbx * g.nby * g.npz * hpmc::kColStride);
     h->n_ctiles = (h->n_cc + hpmc::kScanTile - 1) / hpmc::kScanTile;
     // a block holds at most 256 * 15 vertices: 15 work items
     h->n_items_max = (uint64_t)g.nbx * g.nby * g.cz * 15;
@@ -456,17 +461,16 @@ bool launch_classify(HPMCHistoPyramid* h, float T)
     const void* fsrc = field_source(h);
     stage_mark(h, kStageClassify);
     if (h->mode == kFieldVolume) {
-        hipLaunchKernelGGL(hpmc_classify_volume, dim3(blocks), dim3(threads), 0, h->stream, g, T, fsrc, h->level4,
-                           h->colcnt, h->zseg, (const uint64_t*)h->constants->d_cases);
+        hipLaunchKernelGGL(hpmc_classify_volume, dim3(blocks), dim3(threads), 0, h->stream, g, T, fsrc, h->colcnt,
+                           h->zseg, (const uint64_t*)h->constants->d_cases);
     } else {
         hpmc::Geom ga = g;
         float Ta = T;
         const void* fa = fsrc;
-        uint32_t* l4 = h->level4;
         uint16_t* aw = h->colcnt;
         int zs = h->zseg;
         const uint64_t* cs = h->constants->d_cases;
-        void* args[] = {&ga, &Ta, &fa, &l4, &aw, &zs, &cs};
+        void* args[] = {&ga, &Ta, &fa, &aw, &zs, &cs};
         HIP_TRY(h, hipModuleLaunchKernel(h->k_classify, blocks, 1, 1, threads, 1, 1, 0, h->stream, args, nullptr));
     }
     HIP_TRY(h, hipGetLastError());
@@ -479,9 +483,9 @@ bool launch_scan(HPMCHistoPyramid* h)
     const uint32_t n = (uint32_t)h->n_level4;
     stage_mark(h, kStageScan);
     const unsigned grid = h->n_tiles + h->n_ctiles;
-    hipLaunchKernelGGL(hpmc_scan_reduce, dim3(grid), dim3(256), 0, h->stream, h->level4, n, h->n_tiles,
+    hipLaunchKernelGGL(hpmc_scan_reduce, dim3(grid), dim3(256), 0, h->stream, h->geom, n, h->n_tiles,
                        (const uint16_t*)h->colcnt, h->n_cc, h->tile_sums);
-    hipLaunchKernelGGL(hpmc_scan_scatter, dim3(grid), dim3(256), 0, h->stream, h->geom, h->level4, n, h->n_tiles,
+    hipLaunchKernelGGL(hpmc_scan_scatter, dim3(grid), dim3(256), 0, h->stream, h->geom, n, h->n_tiles,
                        (const uint16_t*)h->colcnt, h->n_cc, (const uint2*)h->tile_sums, h->n_ctiles, h->l4_off,
                        h->items, h->d_totals, h->h_totals);
     HIP_TRY(h, hipGetLastError());
@@ -888,9 +892,10 @@ uint64_t HPMCgetLevel4(struct HPMCHistoPyramid* h, uint32_t* out, uint64_t capac
         fail(h, HPMC_ERR_CAPACITY, "need %llu entries", (unsigned long long)h->n_level4);
         return 0;
     }
-    if (hipMemcpyAsync(out, h->level4, h->n_level4 * sizeof(uint32_t), hipMemcpyDeviceToDevice, h->stream) !=
-        hipSuccess) {
-        fail(h, HPMC_ERR_HIP, "copy failed");
+    hipLaunchKernelGGL(hpmc_level4_gather, dim3(1024), dim3(256), 0, h->stream, h->geom,
+                       (const uint16_t*)h->colcnt, (uint32_t)h->n_level4, out);
+    if (hipGetLastError() != hipSuccess) {
+        fail(h, HPMC_ERR_HIP, "level-4 gather launch failed");
         return 0;
     }
     ok(h);

@@ -48,9 +48,15 @@ struct Geom {
     int blocks_log2;      // log2 of the level-4 side (l2 + 1 - kBlockLog2)
     float scx, scy, scz;  // extent / cells
     float ipx, ipy, ipz;  // 1 / lattice size (snippet texel-centre coordinates)
-    int zseg;             // classification slices per workgroup
+    int zseg;             // classification slices per workgroup: 8, 16 or 32
     int nbx, nby, npz;    // level-4 block columns along x, y; z segments
 };
+
+// Per (z segment, y, x) block column the vertex counts of its slices, 32
+// u16 entries (64 bytes) per column whatever the segment length, so each
+// column is written as one full line.
+constexpr int kColStride = 32;
+
 
 struct Totals {
     uint32_t vertices;    // N
@@ -94,6 +100,29 @@ __device__ __forceinline__ uint32_t block_key(const Geom& g, int x0, int y0, int
     return morton_spread(X) | (morton_spread(Y) << 1);
 }
 
+// Vertex count of the level-4 block with Morton key `key` (0 outside the
+// grid), read from the column-major count array.
+__device__ __forceinline__ uint32_t key_count(const Geom& g, const uint16_t* __restrict__ colcnt, uint32_t key)
+{
+    const int X = (int)morton_compact(key) << kBlockLog2;
+    const int Y = (int)morton_compact(key >> 1) << kBlockLog2;
+    const int tx = X / g.tsx2, ty = Y / g.tsy2;
+    const int z = ty * g.tiles_x + tx;
+    const int bx = (X - tx * g.tsx2) >> kBlockLog2, by = (Y - ty * g.tsy2) >> kBlockLog2;
+    if (tx >= g.tiles_x || z >= g.cz || bx >= g.nbx || by >= g.nby) return 0u;
+    const int pz = z / g.zseg, zi = z - pz * g.zseg;
+    return colcnt[(((size_t)pz * g.nby + by) * g.nbx + bx) * kColStride + zi];
+}
+
+// Orders this wave's LDS accesses around a phase change (each wave owns its
+// LDS region; no other wave takes part).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
 // Field sources.  at(i,j,k) returns the sample at lattice index (i,j,k).
 
@@ -134,6 +163,17 @@ struct VolumeField {
     {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p, idx, 0, 0));
     }
+    // Row-wise access: a per-lane column offset and a wave-uniform row
+    // offset (a scalar register of the load).
+    __device__ __forceinline__ uint32_t col_index(int i) const { return (uint32_t)min(max(i, 0), sx - 1) * 4u; }
+    __device__ __forceinline__ uint32_t row_index(int j) const
+    {
+        return (uint32_t)min(max(j, 0), sy - 1) * (uint32_t)sx * 4u;
+    }
+    __device__ __forceinline__ float fetch2(Plane p, uint32_t col, uint32_t row) const
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p, col, row, 0));
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -155,13 +195,13 @@ struct VolumeField {
 constexpr int kClassifyMaxWaves = 16;
 
 template <class F>
-__device__ __forceinline__ void classify_load_plane(const F& field, const uint32_t (&roff)[kBlock + 1],
+__device__ __forceinline__ void classify_load_plane(const F& field, uint32_t col, const uint32_t (&row)[kBlock + 1],
                                                     uint32_t eoff, bool load_e, int zz, float (&v)[kBlock + 1],
                                                     float& ve)
 {
     const typename F::Plane pl = field.plane(zz);
 #pragma unroll
-    for (int r = 0; r <= kBlock; ++r) v[r] = field.fetch(pl, roff[r]);
+    for (int r = 0; r <= kBlock; ++r) v[r] = field.fetch2(pl, col, row[r]);
     if (load_e) ve = field.fetch(pl, eoff);
 }
 
@@ -250,9 +290,10 @@ __device__ __forceinline__ void classify_tile(const Geom& g, int zseg, int wgx, 
 
 template <class F>
 __device__ __forceinline__ void classify_body(const F& field, const Geom& g, float T,
-                                              uint32_t* __restrict__ level4, uint16_t* __restrict__ colcnt,
-                                              int zseg, const uint8_t* __restrict__ lut,
-                                              uint32_t (*xch)[kClassifyMaxWaves])
+                                              uint16_t* __restrict__ colcnt, int zseg,
+                                              const uint8_t* __restrict__ lut,
+                                              uint32_t (*xch)[kClassifyMaxWaves],
+                                              uint16_t (*stage)[4][kColStride])
 {
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -281,29 +322,32 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
     w.lane_e = load_e && lane <= kBlock && (w.y0 + lane < g.sy);
     const int z0 = pz * zseg;
     const int z1 = min(z0 + zseg, g.cz);   // cell slices [z0, z1): planes z0..z1
-    // per-lane plane offsets of the 17 rows (clamped) and of the column right
-    // of the workgroup
+    // row offsets of the 17 rows (clamped, wave-uniform), the lane's column
+    // offset, and the per-lane offset into the column right of the workgroup
     uint32_t roff[kBlock + 1];
 #pragma unroll
-    for (int r = 0; r <= kBlock; ++r) roff[r] = field.lane_index(w.x, w.y0 + r);
+    for (int r = 0; r <= kBlock; ++r) roff[r] = __builtin_amdgcn_readfirstlane(field.row_index(w.y0 + r));
+    const uint32_t coff = field.col_index(w.x);
     const uint32_t eoff = field.lane_index(w.x0 + 64, w.y0 + min(lane, kBlock));
     const uint64_t erows = (1ull << (kBlock + 1)) - 1ull;
 
-    // Two row buffers: the current plane and the next plane in flight.  The
+    // Two row buffers: the current plane and the next one in flight.  The
     // prefetch is unconditional (the last one re-reads plane z1) so the wait
     // for one buffer never covers the other's loads.  On the common path a
     // plane costs 17 compares folded into per-lane column bits and two
     // ballots; codes are formed only for slices whose planes are mixed.
     const uint32_t full = (1u << (kBlock + 1)) - 1u;
-    // this lane's block column in the column-major count array (lanes 0, 16,
-    // 32, 48 own one block column each)
-    uint16_t* ccol = colcnt + (((size_t)pz * g.nby + py) * g.nbx + (w.x >> kBlockLog2)) * g.zseg;
+    // the four block columns' slice counts are staged in LDS and written as
+    // whole 64-byte lines after the last slice
+    uint16_t(&mycnt)[4][kColStride] = stage[wv];
+    mycnt[lane >> 4][(lane & 15) * 2] = 0;
+    mycnt[lane >> 4][(lane & 15) * 2 + 1] = 0;
     float A[kBlock + 1], B[kBlock + 1], ea = 0.0f, eb_ = 0.0f;
     ClassifyCarry c{};
-    classify_load_plane(field, roff, eoff, load_e, z0, A, ea);
+    classify_load_plane(field, coff, roff, eoff, load_e, z0, A, ea);
     auto step = [&](const float(&cur)[kBlock + 1], float& cure, float(&nxt)[kBlock + 1], float& nxte, int zz,
                     int parity) {
-        classify_load_plane(field, roff, eoff, load_e, min(zz + 1, z1), nxt, nxte);
+        classify_load_plane(field, coff, roff, eoff, load_e, min(zz + 1, z1), nxt, nxte);
         __builtin_amdgcn_sched_barrier(0);
         const uint32_t col = classify_column(w, T, cur);
         uint64_t any = __ballot(col != 0);
@@ -322,8 +366,7 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
             const bool uniform_out = (c.any | any) == 0ull;
             const bool uniform_in = (c.all & all) == ~0ull;
             if (!(uniform_out || uniform_in)) acc = classify_slice_counts(w, c.col, col, c.eb, eb, lut);
-            if ((lane & 15) == 0 && w.x < w.cx) level4[block_key(g, w.x, w.y0, zz - 1)] = acc;
-            if ((lane & 15) == 0 && w.x < w.cx) ccol[zz - 1 - z0] = (uint16_t)acc;
+            if ((lane & 15) == 0) mycnt[lane >> 4][zz - 1 - z0] = (uint16_t)acc;
         }
         c.col = col;
         c.eb = eb;
@@ -335,6 +378,15 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
         if (++zz > z1) break;
         step(B, eb_, A, ea, zz, 1);
         if (++zz > z1) break;
+    }
+    // lane l writes entries 2(l%16), 2(l%16)+1 of block column l/16
+    wave_lds_sync();
+    const int bcol = (w.x0 >> kBlockLog2) + (lane >> 4);
+    if (bcol < g.nbx) {
+        const uint32_t pair = (uint32_t)mycnt[lane >> 4][(lane & 15) * 2] |
+                              ((uint32_t)mycnt[lane >> 4][(lane & 15) * 2 + 1] << 16);
+        uint32_t* col32 = (uint32_t*)(colcnt + (((size_t)pz * g.nby + py) * g.nbx + bcol) * kColStride);
+        col32[lane & 15] = pair;
     }
 }
 
@@ -413,10 +465,19 @@ __device__ __forceinline__ void scan_load16_u16(const uint16_t* __restrict__ cc,
     }
 }
 
-// Tiles [0, ntiles) cover the level-4 array (Morton order): sums of counts
+// The 16 level-4 counts of thread t of a tile of Morton keys, gathered from
+// the column-major counts (all 16 loads in flight together).
+__device__ __forceinline__ void scan_gather16(const Geom& g, const uint16_t* __restrict__ colcnt, uint32_t n,
+                                              uint32_t base, uint32_t (&c)[kScanPer])
+{
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) c[k] = (base + k < n) ? key_count(g, colcnt, base + k) : 0u;
+}
+
+// Tiles [0, ntiles) cover the level-4 keys (Morton order): sums of counts
 // (x) and of non-empty blocks (y).  Tiles [ntiles, ntiles + nctiles) cover
 // the column-major counts: vertices (x) and emit work items (y).
-__device__ __forceinline__ void scan_reduce_body(const uint32_t* __restrict__ level4, uint32_t n, uint32_t ntiles,
+__device__ __forceinline__ void scan_reduce_body(const Geom& g, uint32_t n, uint32_t ntiles,
                                                  const uint16_t* __restrict__ colcnt, uint32_t ncc,
                                                  uint2* __restrict__ tile_sums)
 {
@@ -424,7 +485,7 @@ __device__ __forceinline__ void scan_reduce_body(const uint32_t* __restrict__ le
     uint2 s = make_uint2(0, 0);
     uint32_t c[kScanPer];
     if (blockIdx.x < ntiles) {
-        scan_load16(level4, n, blockIdx.x * kScanTile + threadIdx.x * kScanPer, c);
+        scan_gather16(g, colcnt, n, blockIdx.x * kScanTile + threadIdx.x * kScanPer, c);
 #pragma unroll
         for (int k = 0; k < kScanPer; ++k) {
             s.x += c[k];
@@ -451,7 +512,7 @@ __device__ __forceinline__ void scan_reduce_body(const uint32_t* __restrict__ le
 // emit work items -- (block key, first vertex | count << 16), one per
 // kEmitChunk vertices of a block -- in (z segment, y, x, z) order, so that
 // consecutive items are consecutive slices of one block column.
-__device__ __forceinline__ void scan_scatter_body(const Geom& g, const uint32_t* __restrict__ level4, uint32_t n,
+__device__ __forceinline__ void scan_scatter_body(const Geom& g, uint32_t n,
                                                   uint32_t ntiles, const uint16_t* __restrict__ colcnt, uint32_t ncc,
                                                   const uint2* __restrict__ tile_sums, uint32_t nctiles,
                                                   uint32_t* __restrict__ l4_off, uint2* __restrict__ items,
@@ -467,7 +528,7 @@ __device__ __forceinline__ void scan_scatter_body(const Geom& g, const uint32_t*
     uint32_t c[kScanPer];
     uint2 mine = make_uint2(0, 0);
     if (!is_col) {
-        scan_load16(level4, n, base, c);
+        scan_gather16(g, colcnt, n, base, c);
 #pragma unroll
         for (int k = 0; k < kScanPer; ++k) mine.x += c[k];
     } else {
@@ -527,8 +588,8 @@ __device__ __forceinline__ void scan_scatter_body(const Geom& g, const uint32_t*
         for (int k = 0; k < kScanPer; ++k) {
             if (!c[k]) continue;
             const uint32_t ci = base + k;
-            const uint32_t wi = ci / (uint32_t)g.zseg;
-            const int zi = (int)(ci - wi * (uint32_t)g.zseg);
+            const uint32_t wi = ci / (uint32_t)kColStride;
+            const int zi = (int)(ci % (uint32_t)kColStride);
             const int bxb = (int)(wi % (uint32_t)g.nbx);
             const uint32_t rest = wi / (uint32_t)g.nbx;
             const int by = (int)(rest % (uint32_t)g.nby);
@@ -573,15 +634,6 @@ struct EmitShared {
     uint8_t lut[256];
     EmitWaveShared w[kEmitWaves];
 };
-
-// Orders this wave's LDS accesses around a phase change (each wave owns its
-// LDS region; no other wave takes part).
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // One level of the in-block descent.  A node holds the inclusive prefix
 // sums of its four children (p0 <= p1 <= p2 <= p3); pick the child whose
@@ -803,6 +855,14 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
     }
 }
 
+// The level-4 counts in Morton key order (parity and debugging).
+__device__ __forceinline__ void level4_body(const Geom& g, const uint16_t* __restrict__ colcnt, uint32_t n,
+                                            uint32_t* __restrict__ out)
+{
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        out[k] = key_count(g, colcnt, k);
+}
+
 // Per-cell codes of the whole grid (parity and debugging).
 template <class F>
 __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float T, uint8_t* __restrict__ codes)
@@ -826,14 +886,15 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
 // object from the Geom `g` and the opaque source pointer `fsrc`.
 #define HPMC_DEFINE_FIELD_KERNELS(SUFFIX, MAKE_FIELD)                                          \
     extern "C" __global__ __launch_bounds__(1024) void hpmc_classify_##SUFFIX(                 \
-        hpmc::Geom g, float T, const void* fsrc, uint32_t* level4, uint16_t* colcnt,          \
-        int zseg, const uint64_t* cases)                                                      \
+        hpmc::Geom g, float T, const void* fsrc, uint16_t* colcnt, int zseg,                  \
+        const uint64_t* cases)                                                                \
     {                                                                                         \
         __shared__ uint8_t lut[256];                                                          \
         __shared__ uint32_t xch[2][hpmc::kClassifyMaxWaves];                                   \
+        __shared__ uint16_t stage[hpmc::kClassifyMaxWaves][4][hpmc::kColStride];               \
         for (int i = threadIdx.x; i < 256; i += blockDim.x) lut[i] = (uint8_t)(cases[i] & 0xfu); \
         __syncthreads();                                                                      \
-        hpmc::classify_body(MAKE_FIELD, g, T, level4, colcnt, zseg, lut, xch);                \
+        hpmc::classify_body(MAKE_FIELD, g, T, colcnt, zseg, lut, xch, stage);                 \
     }                                                                                         \
     extern "C" __global__ __launch_bounds__(256) void hpmc_emit_##SUFFIX(                      \
         hpmc::Geom g, float T, const void* fsrc, const uint2* items, const uint32_t* l4_off,   \
