@@ -26,6 +26,10 @@ sys.path.insert(0, ROOT)
 METRIC = "cells/s + verts/s at 512^3 fp32, % HBM peak; batched/plain speedup"
 HBM_PEAK_GBS = 8000.0
 N_CAYLEY_512 = 3694104
+# HBM traffic of the classification kernel per launch, from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE runs of this script, scaled by the same counters on
+# a plain copy of known size (bench.py --copy-calibration)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hpmc_classify_volume_512_traffic.json")
 
 
 def cayley_volume_torch(n, device):
@@ -60,6 +64,33 @@ def cpu_baseline(n, reps):
             "sample": "full %d^3 Cayley volume, %d runs (median %.2f s), %d vertices" % (n, reps, t, nv)}
 
 
+def copy_calibration(steps):
+    """A plain 512 MiB device copy (torch copy_), for scaling the HBM counters:
+    each launch reads and writes exactly 536,870,912 bytes."""
+    import torch
+    x = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    print(json.dumps({"copy_bytes_each_way": x.numel() * 4, "launches": steps, "ms_per_copy": ms,
+                      "rw_GBps": 2 * x.numel() * 4 / (ms * 1e-3) / 1e9}))
+
+
+def traffic():
+    """HBM bytes per classify launch from the committed counter profile of this
+    workload, or None when the profile is absent."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            d = json.load(fh)
+        return d["fetch_bytes"] + d["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,12 +98,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="time a plain 512 MiB copy instead (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the engine builds one volume per GPU with no "
               "collectives; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.steps)
+        return
 
     import torch
     import hpmc_amd
@@ -145,7 +182,7 @@ def main():
         "ms_per_step_with_stage_events": (t1 - t0) * 1e3 / args.steps,
         "roofline": {"kernel": "hpmc_classify_volume", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None},
+                     "traffic": traffic() if n == 512 else None, "algorithmic_bytes": lattice_bytes},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_reps)
