@@ -67,7 +67,7 @@ SIGNATURES = [
     ("HPMCgetLevel4", _u64, [_vp, _vp, _u64]),
     ("HPMCsetProfiling", _i32, [_vp, _i32]),
     ("HPMCgetStageTimes", _i32, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u32)]),
-    ("HPMCcheckFieldSnippet", _i32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
+    ("HPMCcheckFieldSnippet", _i32, [ctypes.c_char_p, _u32, ctypes.c_char_p, ctypes.c_size_t]),
     ("HPMCgetLayout", _i32, [_vp, ctypes.POINTER(_i32)]),
 ]
 
@@ -105,10 +105,11 @@ def _check(code, h=None):
         raise HPMCError("%s: %s" % (ERRORS.get(code, code), msg))
 
 
-def check_snippet(source):
+def check_snippet(source, gradient=False, binary=False):
     """Compile a field snippet for gfx950 without a device; returns the log."""
     buf = ctypes.create_string_buffer(1 << 16)
-    code = lib().HPMCcheckFieldSnippet(source.encode(), buf, len(buf))
+    flags = (1 if gradient else 0) | (2 if binary else 0)
+    code = lib().HPMCcheckFieldSnippet(source.encode(), flags, buf, len(buf))
     _check(code)
     return buf.value.decode()
 
@@ -151,16 +152,29 @@ class IsoSurface:
         self._ce = None
 
     # -- field ---------------------------------------------------------------
-    def set_volume(self, volume):
-        """volume: contiguous float32 device tensor shaped (z, y, x)."""
+    def set_volume(self, volume, gradient=False):
+        """volume: contiguous float32 device tensor shaped (z, y, x), or with
+        gradient=True (z, y, x, 4) holding (gradient xyz, value) per lattice
+        point like an RGBA gradient texture."""
         import torch
         assert volume.dtype == torch.float32 and volume.is_contiguous() and volume.is_cuda
-        assert tuple(volume.shape) == self.lattice[::-1], (volume.shape, self.lattice)
+        shape = self.lattice[::-1] + ((4,) if gradient else ())
+        assert tuple(volume.shape) == shape, (volume.shape, shape)
         self._volume = volume
-        lib().HPMCsetFieldTexture3D(self.h, ctypes.c_void_p(volume.data_ptr()), 0)
+        lib().HPMCsetFieldTexture3D(self.h, ctypes.c_void_p(volume.data_ptr()), int(bool(gradient)))
 
-    def set_snippet(self, source):
-        lib().HPMCsetFieldCustom(self.h, source.encode(), 0, 0)
+    def set_snippet(self, source, gradient=False):
+        """HIP C++ defining HPMC_fetch(float3) (and HPMC_fetchGrad(float3) ->
+        float4 (gradient, value) with gradient=True)."""
+        lib().HPMCsetFieldCustom(self.h, source.encode(), 0, int(bool(gradient)))
+
+    def set_binary(self, binary=True):
+        """Binary field: classify against 0.5, emit edge midpoints and face
+        normals.  As in HPMC, the flag takes effect at the next
+        reconfiguration (the first build, or after a setter that changes the
+        lattice, grid, extent or field mode)."""
+        L = lib()
+        (L.HPMCsetFieldAsBinary if binary else L.HPMCsetFieldAsContinuous)(self.h)
 
     def set_uniform(self, name, array):
         import numpy as np

@@ -27,7 +27,14 @@
 // Device sources embedded for hipRTC (generated by hpmc_amd/build.py).
 #include "_embedded_sources.inc"
 
-HPMC_DEFINE_FIELD_KERNELS(volume, (hpmc::VolumeField{(const float*)fsrc, g.sx, g.sy, g.sz}))
+#define HPMC_VOLUME_FIELD (hpmc::VolumeField{(const float*)fsrc, g.sx, g.sy, g.sz})
+#define HPMC_VOLUME4_FIELD (hpmc::Volume4Field{(const float4*)fsrc, g.sx, g.sy, g.sz})
+HPMC_DEFINE_BUILD_KERNELS(volume, HPMC_VOLUME_FIELD)
+HPMC_DEFINE_BUILD_KERNELS(volume4, HPMC_VOLUME4_FIELD)
+HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume, HPMC_VOLUME_FIELD, hpmc::kEmitContinuous)
+HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume_binary, HPMC_VOLUME_FIELD, hpmc::kEmitBinary)
+HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume4_gradient, HPMC_VOLUME4_FIELD, hpmc::kEmitGradient)
+HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume4_binary, HPMC_VOLUME4_FIELD, hpmc::kEmitBinary)
 
 extern "C" __global__ __launch_bounds__(256) void hpmc_scan_reduce(hpmc::Geom g, uint32_t n, uint32_t ntiles,
                                                                    const uint16_t* colcnt, uint32_t ncc,
@@ -55,6 +62,60 @@ namespace {
 
 const uint64_t kCases[256] = HPMC_MC_CASES_INIT;
 
+#pragma clang fp contract(off)
+// Per HPMC code and slot, the normal a binary field's vertex carries
+// (constants.cpp:217-273 builds it; shadergen.cpp:507-510 decodes it): the
+// face normal n = (m2 - m0) x (m1 - m0) of the slot's triangle over the edge
+// midpoints m0..m2, scaled by 0.5 |n|, offset by 0.5, clamped into (0, 1),
+// added to the slot's integer edge shift and decoded as 2 fract(.) - 1.
+// FP32 throughout, in the reference's operation order.
+void binary_normals(float4 (&out)[256 * 16])
+{
+    for (int c = 0; c < 256; ++c) {
+        const int count = (int)(kCases[c] & 0xfu);
+        float mid[15][3], shift[15][3];
+        for (int i = 0; i < count; ++i) {
+            const uint32_t e = (uint32_t)(kCases[c] >> (4 + 4 * i)) & 0xfu;
+            const int axis = (int)(e >> 2), s0 = (int)(e & 1u), s1 = (int)((e >> 1) & 1u);
+            float sh[3] = {0.0f, 0.0f, 0.0f};
+            if (axis == 0) { sh[1] = (float)s0; sh[2] = (float)s1; }
+            else if (axis == 1) { sh[0] = (float)s0; sh[2] = (float)s1; }
+            else { sh[0] = (float)s0; sh[1] = (float)s1; }
+            for (int k = 0; k < 3; ++k) {
+                shift[i][k] = sh[k];
+                mid[i][k] = sh[k] + (k == axis ? 0.5f : 0.0f);
+            }
+        }
+        for (int i = 0; i < 16; ++i) out[c * 16 + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int i = 0; i < count; ++i) {
+            const int t = i / 3;
+            const float* m0 = mid[3 * t];
+            const float* m1 = mid[3 * t + 1];
+            const float* m2 = mid[3 * t + 2];
+            float u[3], v[3];
+            for (int k = 0; k < 3; ++k) {
+                u[k] = m2[k] - m0[k];
+                v[k] = m1[k] - m0[k];
+            }
+            float n[3];
+            n[0] = u[1] * v[2] - u[2] * v[1];
+            n[1] = u[2] * v[0] - u[0] * v[2];
+            n[2] = u[0] * v[1] - u[1] * v[0];
+            const float s = 0.5f * sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            float r[3];
+            for (int k = 0; k < 3; ++k) {
+                float tmp = s * n[k] + 0.5f;
+                if (tmp < 0.0f) tmp = 0.000001f;
+                if (tmp >= 1.0f) tmp = 0.999999f;
+                const float packed = shift[i][k] + tmp;
+                r[k] = 2.0f * (packed - floorf(packed)) - 1.0f;
+            }
+            out[c * 16 + i] = make_float4(r[0], r[1], r[2], 0.0f);
+        }
+    }
+}
+#pragma clang fp contract(on)
+
 enum FieldMode { kFieldVolume = 0, kFieldCustom = 1 };
 
 thread_local int32_t t_last_error = HPMC_OK;
@@ -65,6 +126,7 @@ thread_local std::string t_last_msg;
 struct HPMCConstants {
     int device = 0;
     uint64_t* d_cases = nullptr;
+    float4* d_bnorm = nullptr;   // binary-field normals by (code, slot)
 };
 
 struct HPMCHistoPyramid {
@@ -74,7 +136,8 @@ struct HPMCHistoPyramid {
     int32_t size[3] = {0, 0, 0};
     int32_t cells[3] = {0, 0, 0};
     float extent[3] = {1.0f, 1.0f, 1.0f};
-    bool binary = false;
+    bool binary = false;             // as last set
+    bool binary_active = false;      // as taken at the last configuration (setup)
     int mode = kFieldVolume;
     const float* volume = nullptr;
     bool gradient = false;
@@ -107,6 +170,10 @@ struct HPMCHistoPyramid {
     // hipRTC module for snippets
     hipModule_t module = nullptr;
     hipFunction_t k_classify = nullptr, k_emit = nullptr, k_codes = nullptr;
+    // built-in kernels for volume fields, chosen at setup
+    const void* f_classify = nullptr;
+    const void* f_emit = nullptr;
+    const void* f_codes = nullptr;
     uint32_t builder_id = 0;
     // optional per-stage timing: event pairs recorded around each launch
     bool profiling = false;
@@ -223,10 +290,11 @@ bool determine_layout(HPMCHistoPyramid* h)
     return true;
 }
 
-std::string rtc_source(const std::string& snippet)
+std::string rtc_source(const std::string& snippet, bool gradient, bool binary)
 {
     std::string s;
     s += "#define HPMC_RTC 1\n";
+    if (gradient) s += "#define HPMC_FIELD_GRADIENT 1\n";
     s += "typedef unsigned char uint8_t;\ntypedef unsigned short uint16_t;\n";
     s += "typedef unsigned int uint32_t;\ntypedef unsigned long long uint64_t;\n";
     s += "#include \"hpmc_device.h\"\n";
@@ -254,9 +322,18 @@ std::string rtc_source(const std::string& snippet)
          "    __device__ __forceinline__ float fetch2(Plane pz, uint32_t col, uint32_t row) const {\n"
          "        return HPMC_fetch(make_float3(((float)col + 0.5f) * ipx, ((float)row + 0.5f) * ipy, pz));\n"
          "    }\n"
+         "#ifdef HPMC_FIELD_GRADIENT\n"
+         "    __device__ __forceinline__ float4 grad(int i, int j, int k) const {\n"
+         "        return HPMC_fetchGrad(make_float3(((float)i + 0.5f) * ipx, ((float)j + 0.5f) * ipy,\n"
+         "                                          ((float)k + 0.5f) * ipz));\n"
+         "    }\n"
+         "#endif\n"
          "};\n"
          "}\n"
-         "HPMC_DEFINE_FIELD_KERNELS(rtc, (hpmc::SnippetField{g.ipx, g.ipy, g.ipz}))\n";
+         "HPMC_DEFINE_BUILD_KERNELS(rtc, (hpmc::SnippetField{g.ipx, g.ipy, g.ipz}))\n";
+    const char* mode = binary ? "hpmc::kEmitBinary" : gradient ? "hpmc::kEmitGradient" : "hpmc::kEmitContinuous";
+    s += std::string("HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_rtc, (hpmc::SnippetField{g.ipx, g.ipy, g.ipz}), ") + mode +
+         ")\n";
     return s;
 }
 
@@ -265,10 +342,10 @@ uint32_t g_builder_ids = 0;
 
 // Compiles the snippet behind the device header; returns the code object
 // or an empty vector with the compiler log in *log.  Needs no device.
-std::vector<char> rtc_compile(const std::string& snippet, std::string* log)
+std::vector<char> rtc_compile(const std::string& snippet, bool gradient, bool binary, std::string* log)
 {
     std::lock_guard<std::mutex> lock(g_rtc_mutex);
-    std::string src = rtc_source(snippet);
+    std::string src = rtc_source(snippet, gradient, binary);
     const char* hdr_src[2] = {kEmbeddedDeviceHeader, kEmbeddedTablesHeader};
     const char* hdr_names[2] = {"hpmc_device.h", "mc_tables.h"};
     hiprtcProgram prog;
@@ -300,7 +377,7 @@ std::vector<char> rtc_compile(const std::string& snippet, std::string* log)
 bool compile_snippet(HPMCHistoPyramid* h)
 {
     std::string log;
-    std::vector<char> code = rtc_compile(h->snippet, &log);
+    std::vector<char> code = rtc_compile(h->snippet, h->gradient, h->binary_active, &log);
     if (code.empty()) {
         fail(h, HPMC_ERR_COMPILE, "field snippet failed to compile: %s", log.c_str());
         return false;
@@ -363,13 +440,20 @@ bool setup(HPMCHistoPyramid* h)
         fail(h, HPMC_ERR_FIELD, "empty field snippet");
         return false;
     }
-    if (h->gradient) {
-        fail(h, HPMC_ERR_UNSUPPORTED, "gradient-providing fields are not supported yet");
-        return false;
-    }
-    if (h->binary) {
-        fail(h, HPMC_ERR_UNSUPPORTED, "binary fields are not supported yet");
-        return false;
+    // hpiface.cpp:131-142: the binary flag does not taint; it takes effect
+    // when the programs are next generated, i.e. here
+    h->binary_active = h->binary;
+    if (h->mode == kFieldVolume) {
+        if (h->gradient) {
+            h->f_classify = (const void*)hpmc_classify_volume4;
+            h->f_codes = (const void*)hpmc_codes_volume4;
+            h->f_emit = h->binary_active ? (const void*)hpmc_emit_volume4_binary
+                                         : (const void*)hpmc_emit_volume4_gradient;
+        } else {
+            h->f_classify = (const void*)hpmc_classify_volume;
+            h->f_codes = (const void*)hpmc_codes_volume;
+            h->f_emit = h->binary_active ? (const void*)hpmc_emit_volume_binary : (const void*)hpmc_emit_volume;
+        }
     }
     hpmc::Geom& g = h->geom;
     g.sx = h->size[0];
@@ -449,6 +533,20 @@ void stage_mark(HPMCHistoPyramid* h, int stage)
 
 const void* field_source(HPMCHistoPyramid* h) { return h->mode == kFieldVolume ? (const void*)h->volume : nullptr; }
 
+// Launches a field kernel: the built-in one for volume fields, the hipRTC
+// one for snippets; `args` point at the kernel's arguments in order.
+bool launch_field_kernel(HPMCHistoPyramid* h, const void* builtin, hipFunction_t rtc, unsigned grid,
+                         unsigned threads, void** args)
+{
+    if (h->mode == kFieldVolume) {
+        HIP_TRY(h, hipLaunchKernel(builtin, dim3(grid), dim3(threads), args, 0, h->stream));
+    } else {
+        HIP_TRY(h, hipModuleLaunchKernel(rtc, grid, 1, 1, threads, 1, 1, 0, h->stream, args, nullptr));
+    }
+    HIP_TRY(h, hipGetLastError());
+    return true;
+}
+
 bool launch_classify(HPMCHistoPyramid* h, float T)
 {
     const hpmc::Geom& g = h->geom;
@@ -458,22 +556,15 @@ bool launch_classify(HPMCHistoPyramid* h, float T)
     const int64_t tiles = wgx * ((g.cy + hpmc::kBlock - 1) / hpmc::kBlock) * ((g.cz + h->zseg - 1) / h->zseg);
     const unsigned blocks = (unsigned)(((tiles + 7) / 8) * 8);
     const unsigned threads = 64u * (unsigned)nw;
-    const void* fsrc = field_source(h);
+    hpmc::Geom ga = g;
+    float Ta = T;
+    const void* fa = field_source(h);
+    uint16_t* aw = h->colcnt;
+    int zs = h->zseg;
+    const uint64_t* cs = h->constants->d_cases;
+    void* args[] = {&ga, &Ta, &fa, &aw, &zs, &cs};
     stage_mark(h, kStageClassify);
-    if (h->mode == kFieldVolume) {
-        hipLaunchKernelGGL(hpmc_classify_volume, dim3(blocks), dim3(threads), 0, h->stream, g, T, fsrc, h->colcnt,
-                           h->zseg, (const uint64_t*)h->constants->d_cases);
-    } else {
-        hpmc::Geom ga = g;
-        float Ta = T;
-        const void* fa = fsrc;
-        uint16_t* aw = h->colcnt;
-        int zs = h->zseg;
-        const uint64_t* cs = h->constants->d_cases;
-        void* args[] = {&ga, &Ta, &fa, &aw, &zs, &cs};
-        HIP_TRY(h, hipModuleLaunchKernel(h->k_classify, blocks, 1, 1, threads, 1, 1, 0, h->stream, args, nullptr));
-    }
-    HIP_TRY(h, hipGetLastError());
+    if (!launch_field_kernel(h, h->f_classify, h->k_classify, blocks, threads, args)) return false;
     stage_mark(h, kStageClassify);
     return true;
 }
@@ -497,27 +588,20 @@ bool launch_emit(HPMCHistoPyramid* h, float* out, hpmc::CellEdge* out_ce, uint32
 {
     const hpmc::Geom& g = h->geom;
     const int grid = emit_grid(h);
-    const void* fsrc = field_source(h);
+    hpmc::Geom ga = g;
+    float Ta = h->threshold;
+    const void* fa = field_source(h);
+    const uint2* li = h->items;
+    const uint32_t* lo = h->l4_off;
+    const hpmc::Totals* tt = h->d_totals;
+    float* o = out;
+    hpmc::CellEdge* oc = out_ce;
+    uint32_t cap = capacity;
+    const uint64_t* cs = h->constants->d_cases;
+    const float4* bn = h->constants->d_bnorm;
+    void* args[] = {&ga, &Ta, &fa, &li, &lo, &tt, &o, &oc, &cap, &cs, &bn};
     stage_mark(h, kStageEmit);
-    if (h->mode == kFieldVolume) {
-        hipLaunchKernelGGL(hpmc_emit_volume, dim3(grid), dim3(256), 0, h->stream, g, h->threshold, fsrc,
-                           (const uint2*)h->items, (const uint32_t*)h->l4_off, (const hpmc::Totals*)h->d_totals,
-                           out, out_ce, capacity, (const uint64_t*)h->constants->d_cases);
-    } else {
-        hpmc::Geom ga = g;
-        float Ta = h->threshold;
-        const void* fa = fsrc;
-        const uint2* li = h->items;
-        const uint32_t* lo = h->l4_off;
-        const hpmc::Totals* tt = h->d_totals;
-        float* o = out;
-        hpmc::CellEdge* oc = out_ce;
-        uint32_t cap = capacity;
-        const uint64_t* cs = h->constants->d_cases;
-        void* args[] = {&ga, &Ta, &fa, &li, &lo, &tt, &o, &oc, &cap, &cs};
-        HIP_TRY(h, hipModuleLaunchKernel(h->k_emit, grid, 1, 1, 256, 1, 1, 0, h->stream, args, nullptr));
-    }
-    HIP_TRY(h, hipGetLastError());
+    if (!launch_field_kernel(h, h->f_emit, h->k_emit, (unsigned)grid, 256, args)) return false;
     stage_mark(h, kStageEmit);
     return true;
 }
@@ -576,10 +660,16 @@ struct HPMCConstants* HPMCcreateConstants(int32_t, int32_t)
 {
     HPMCConstants* c = new (std::nothrow) HPMCConstants;
     if (!c) return nullptr;
+    static float4 bnorm[256 * 16];
+    static std::once_flag bnorm_once;
+    std::call_once(bnorm_once, [] { binary_normals(bnorm); });
     if (hipGetDevice(&c->device) != hipSuccess || hipMalloc(&c->d_cases, sizeof kCases) != hipSuccess ||
-        hipMemcpy(c->d_cases, kCases, sizeof kCases, hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(c->d_cases, kCases, sizeof kCases, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&c->d_bnorm, sizeof bnorm) != hipSuccess ||
+        hipMemcpy(c->d_bnorm, bnorm, sizeof bnorm, hipMemcpyHostToDevice) != hipSuccess) {
         fail(nullptr, HPMC_ERR_HIP, "failed to upload marching-cubes tables");
         if (c->d_cases) (void)hipFree(c->d_cases);
+        if (c->d_bnorm) (void)hipFree(c->d_bnorm);
         delete c;
         return nullptr;
     }
@@ -591,6 +681,7 @@ void HPMCdestroyConstants(struct HPMCConstants* c)
 {
     if (!c) return;
     if (c->d_cases) (void)hipFree(c->d_cases);
+    if (c->d_bnorm) (void)hipFree(c->d_bnorm);
     delete c;
 }
 
@@ -701,7 +792,7 @@ void HPMCbuildHistopyramid(struct HPMCHistoPyramid* h, float threshold)
     if (!h || h->broken) return;
     // hpiface.cpp:258-268: a failed setup leaves the HP tainted, not broken.
     if (h->tainted && !setup(h)) return;
-    h->threshold = h->binary ? 0.5f : threshold;
+    h->threshold = h->binary_active ? 0.5f : threshold;
     h->count_valid = false;
     h->built = false;
     if (!launch_classify(h, h->threshold) || !launch_scan(h)) {
@@ -861,19 +952,12 @@ int32_t HPMCgetCellCodes(struct HPMCHistoPyramid* h, uint8_t* codes)
     const hpmc::Geom& g = h->geom;
     const uint64_t n = (uint64_t)g.cx * g.cy * g.cz;
     const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 65536);
-    if (h->mode == kFieldVolume) {
-        hipLaunchKernelGGL(hpmc_codes_volume, dim3(blocks), dim3(256), 0, h->stream, g, h->threshold,
-                           field_source(h), codes);
-    } else {
-        hpmc::Geom ga = g;
-        float Ta = h->threshold;
-        const void* fa = nullptr;
-        uint8_t* c = codes;
-        void* args[] = {&ga, &Ta, &fa, &c};
-        if (hipModuleLaunchKernel(h->k_codes, blocks, 1, 1, 256, 1, 1, 0, h->stream, args, nullptr) != hipSuccess)
-            return fail(h, HPMC_ERR_HIP, "codes kernel launch failed");
-    }
-    if (hipGetLastError() != hipSuccess) return fail(h, HPMC_ERR_HIP, "codes kernel launch failed");
+    hpmc::Geom ga = g;
+    float Ta = h->binary_active ? 0.5f : h->threshold;
+    const void* fa = field_source(h);
+    uint8_t* c = codes;
+    void* args[] = {&ga, &Ta, &fa, &c};
+    if (!launch_field_kernel(h, h->f_codes, h->k_codes, blocks, 256, args)) return h->last_error;
     ok(h);
     return HPMC_OK;
 }
@@ -934,11 +1018,12 @@ int32_t HPMCgetStageTimes(struct HPMCHistoPyramid* h, double* ms, uint32_t* laun
     return HPMC_OK;
 }
 
-int32_t HPMCcheckFieldSnippet(const char* source, char* log, size_t log_bytes)
+int32_t HPMCcheckFieldSnippet(const char* source, uint32_t flags, char* log, size_t log_bytes)
 {
     if (!source) return fail(nullptr, HPMC_ERR_NULL, "source == NULL");
     std::string text;
-    std::vector<char> code = rtc_compile(source, &text);
+    std::vector<char> code =
+        rtc_compile(source, (flags & HPMC_SNIPPET_GRADIENT) != 0, (flags & HPMC_SNIPPET_BINARY) != 0, &text);
     if (log && log_bytes) {
         size_t n = text.size() < log_bytes - 1 ? text.size() : log_bytes - 1;
         memcpy(log, text.data(), n);

@@ -17,7 +17,9 @@
 //   emit      : per non-empty block, re-classify its 256 cells from an
 //               LDS-staged 18x18x3 brick, descend the in-block levels 0..3
 //               per output vertex, and write the edge-interpolated position
-//               and forward-difference normal (shadergen.cpp:505-563).
+//               and normal (shadergen.cpp:505-563): forward differences,
+//               the field's own gradient, or a binary field's midpoint and
+//               tabulated face normal.
 // Vertices come out in the reference's extraction order.
 #pragma once
 
@@ -169,6 +171,50 @@ struct VolumeField {
     __device__ __forceinline__ uint32_t row_index(int j) const
     {
         return (uint32_t)min(max(j, 0), sy - 1) * (uint32_t)sx * 4u;
+    }
+    __device__ __forceinline__ float fetch2(Plane p, uint32_t col, uint32_t row) const
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p, col, row, 0));
+    }
+};
+
+// RGBA FP32 volume carrying the gradient in rgb and the scalar in alpha, the
+// layout of a gradient-providing 3D texture (hpiface.cpp:164-184,
+// shadergen.cpp:256-266).  Classification reads the alpha channel; grad()
+// returns the whole texel (gradient, value).
+struct Volume4Field {
+    const float4* __restrict__ v;
+    int sx, sy, sz;
+    __device__ __forceinline__ size_t texel(int i, int j, int k) const
+    {
+        i = min(max(i, 0), sx - 1);
+        j = min(max(j, 0), sy - 1);
+        k = min(max(k, 0), sz - 1);
+        return ((size_t)k * sy + j) * sx + i;
+    }
+    __device__ __forceinline__ float at(int i, int j, int k) const { return v[texel(i, j, k)].w; }
+    __device__ __forceinline__ float4 grad(int i, int j, int k) const { return v[texel(i, j, k)]; }
+    typedef __amdgpu_buffer_rsrc_t Plane;
+    __device__ __forceinline__ uint32_t lane_index(int i, int j) const
+    {
+        i = min(max(i, 0), sx - 1);
+        j = min(max(j, 0), sy - 1);
+        return ((uint32_t)j * (uint32_t)sx + (uint32_t)i) * 16u + 12u;
+    }
+    __device__ __forceinline__ Plane plane(int k) const
+    {
+        k = min(max(k, 0), sz - 1);
+        const float4* base = v + (size_t)k * sy * sx;
+        return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, sx * sy * 16, 0x00020000);
+    }
+    __device__ __forceinline__ float fetch(Plane p, uint32_t idx) const
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p, idx, 0, 0));
+    }
+    __device__ __forceinline__ uint32_t col_index(int i) const { return (uint32_t)min(max(i, 0), sx - 1) * 16u + 12u; }
+    __device__ __forceinline__ uint32_t row_index(int j) const
+    {
+        return (uint32_t)min(max(j, 0), sy - 1) * (uint32_t)sx * 16u;
     }
     __device__ __forceinline__ float fetch2(Plane p, uint32_t col, uint32_t row) const
     {
@@ -620,6 +666,17 @@ __device__ __forceinline__ void scan_scatter_body(const Geom& g, uint32_t n,
 constexpr int kEmitWaves = 4;
 constexpr int kPlaneLoads = (kBrickPlane + 63) / 64;   // 6 per lane per plane
 
+// Emission modes (shadergen.cpp:505-563):
+//   continuous: t from the two edge samples, normal from forward differences
+//   binary:     edge midpoint, normal from the per-case table (threshold 0.5)
+//   gradient:   t from the two samples' values, normal from their gradients
+enum EmitMode { kEmitContinuous = 0, kEmitBinary = 1, kEmitGradient = 2 };
+
+// Lattice planes an emit item stages: z, z+1 for the corners, z+2 for the
+// forward differences of continuous mode.
+template <int MODE>
+constexpr int emit_planes() { return MODE == kEmitContinuous ? 3 : 2; }
+
 struct EmitWaveShared {
     float plane[3][kBrickPlane];   // ring: plane p in slot p % 3
     uint16_t l0[256];              // per-cell vertex counts (Morton order)
@@ -698,13 +755,14 @@ __device__ __forceinline__ void emit_store(EmitWaveShared& ws, int p_lo, int p_h
     }
 }
 
-template <class F>
+template <int MODE, class F>
 __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T,
                                           const uint2* __restrict__ items, const uint32_t* __restrict__ l4_off,
                                           const Totals* __restrict__ totals,
                                           float* __restrict__ out, CellEdge* __restrict__ out_ce,
-                                          uint32_t capacity, EmitShared& sh)
+                                          uint32_t capacity, const float4* __restrict__ bnorm, EmitShared& sh)
 {
+    constexpr int NP = emit_planes<MODE>();
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const uint32_t nitems = __builtin_amdgcn_readfirstlane(totals->items);
@@ -730,7 +788,7 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
     float r[3][kPlaneLoads];
     int x0, y0, z;
     block_origin(g, cur.x, x0, y0, z);
-    emit_fetch(field, x0, y0, z, z + 2, lane, r);
+    emit_fetch(field, x0, y0, z, z + NP - 1, lane, r);
     uint2 nxt = items[min(it + stride, nitems - 1)];
 
     for (;;) {
@@ -738,7 +796,7 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
         const uint32_t base = __builtin_amdgcn_readfirstlane(obase);
         const int cz = z, cx0 = x0, cy0 = y0;
         wave_lds_sync();
-        emit_store(ws, 0, 2, lane, r);
+        emit_store(ws, 0, NP - 1, lane, r);
         wave_lds_sync();
 
         // next item: entry, output offset and planes in flight meanwhile
@@ -749,13 +807,13 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
             nxt.y = __builtin_amdgcn_readfirstlane(nxt.y);
             obase = l4_off[nxt.x];
             block_origin(g, nxt.x, x0, y0, z);
-            emit_fetch(field, x0, y0, z, z + 2, lane, r);
+            emit_fetch(field, x0, y0, z, z + NP - 1, lane, r);
             after = items[min(it + 2 * stride, nitems - 1)];
         }
 
         const float* P0 = ws.plane[0];
         const float* P1 = ws.plane[1];
-        const float* P2 = ws.plane[2];
+        const float* P2 = ws.plane[NP - 1];
         const bool qvx0 = cx0 + qx < g.cx, qvx1 = cx0 + qx + 1 < g.cx;
         const bool qvy0 = cy0 + qy < g.cy, qvy1 = cy0 + qy + 1 < g.cy;
 
@@ -821,24 +879,46 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
             else if (axis == 1) { ax += s0; az = s1; }
             else { ax += s0; ay += s1; }
             const int bx_ = ax + (axis == 0), by_ = ay + (axis == 1), bz_ = az + (axis == 2);
-            const float* Pa = az ? P1 : P0;
-            const float* Pan = az ? P2 : P1;
-            const float* Pb = bz_ ? P1 : P0;
-            const float* Pbn = bz_ ? P2 : P1;
-            const int ia = ay * kBrick + ax, ib = by_ * kBrick + bx_;
-            const float va = Pa[ia], vb = Pb[ib];
-            const float nax = Pa[ia + 1], nay = Pa[ia + kBrick], naz = Pan[ia];
-            const float nbx_ = Pb[ib + 1], nby_ = Pb[ib + kBrick], nbz_ = Pbn[ib];
-            // va and vb lie on opposite sides of T, so va - vb != 0; the
-            // hardware reciprocal (1 ulp) keeps t within a few ulp
-            const float tt = (va - T) * __builtin_amdgcn_rcpf(va - vb);
-            const float u = 1.0f - tt;
             const float gx = (float)(cx0 + ax), gy = (float)(cy0 + ay), gz = (float)(cz + az);
+            float nx, ny, nz, tt;
+            if constexpr (MODE == kEmitBinary) {
+                // shadergen.cpp:507-510, 521-522: midpoint, tabulated normal
+                const float4 bn = bnorm[ws.code[c] * 16u + slot];
+                tt = 0.5f;
+                nx = bn.x;
+                ny = bn.y;
+                nz = bn.z;
+            } else if constexpr (MODE == kEmitGradient) {
+                // shadergen.cpp:539-549: values and gradients at both ends
+                const float4 fa = field.grad(cx0 + ax, cy0 + ay, cz + az);
+                const float4 fb = field.grad(cx0 + bx_, cy0 + by_, cz + bz_);
+                tt = (fa.w - T) * __builtin_amdgcn_rcpf(fa.w - fb.w);
+                const float u = 1.0f - tt;
+                nx = T - (fa.x * u + fb.x * tt);
+                ny = T - (fa.y * u + fb.y * tt);
+                nz = T - (fa.z * u + fb.z * tt);
+            } else {
+                const float* Pa = az ? P1 : P0;
+                const float* Pan = az ? P2 : P1;
+                const float* Pb = bz_ ? P1 : P0;
+                const float* Pbn = bz_ ? P2 : P1;
+                const int ia = ay * kBrick + ax, ib = by_ * kBrick + bx_;
+                const float va = Pa[ia], vb = Pb[ib];
+                const float nax = Pa[ia + 1], nay = Pa[ia + kBrick], naz = Pan[ia];
+                const float nbx_ = Pb[ib + 1], nby_ = Pb[ib + kBrick], nbz_ = Pbn[ib];
+                // va and vb lie on opposite sides of T, so va - vb != 0; the
+                // hardware reciprocal (1 ulp) keeps t within a few ulp
+                tt = (va - T) * __builtin_amdgcn_rcpf(va - vb);
+                const float u = 1.0f - tt;
+                nx = T - (nax * u + nbx_ * tt);
+                ny = T - (nay * u + nby_ * tt);
+                nz = T - (naz * u + nbz_ * tt);
+            }
             const uint32_t o = base + v;
             if (o < capacity) {
                 float2* dst = (float2*)(out + (size_t)o * 6);
-                dst[0] = make_float2((T - (nax * u + nbx_ * tt)) * g.scx, (T - (nay * u + nby_ * tt)) * g.scy);
-                dst[1] = make_float2((T - (naz * u + nbz_ * tt)) * g.scz, (gx + (axis == 0 ? tt : 0.0f)) * g.scx);
+                dst[0] = make_float2(nx * g.scx, ny * g.scy);
+                dst[1] = make_float2(nz * g.scz, (gx + (axis == 0 ? tt : 0.0f)) * g.scx);
                 dst[2] = make_float2((gy + (axis == 1 ? tt : 0.0f)) * g.scy, (gz + (axis == 2 ? tt : 0.0f)) * g.scz);
                 if (out_ce) {
                     CellEdge ce;
@@ -884,7 +964,9 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
 
 // Kernel entry points for one field source.  MAKE_FIELD builds the field
 // object from the Geom `g` and the opaque source pointer `fsrc`.
-#define HPMC_DEFINE_FIELD_KERNELS(SUFFIX, MAKE_FIELD)                                          \
+// HPMC_DEFINE_BUILD_KERNELS: classification and the per-cell code readback.
+// HPMC_DEFINE_EMIT_KERNEL: one extraction kernel in emission mode MODE.
+#define HPMC_DEFINE_BUILD_KERNELS(SUFFIX, MAKE_FIELD)                                          \
     extern "C" __global__ __launch_bounds__(1024) void hpmc_classify_##SUFFIX(                 \
         hpmc::Geom g, float T, const void* fsrc, uint16_t* colcnt, int zseg,                  \
         const uint64_t* cases)                                                                \
@@ -896,19 +978,23 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
         __syncthreads();                                                                      \
         hpmc::classify_body(MAKE_FIELD, g, T, colcnt, zseg, lut, xch, stage);                 \
     }                                                                                         \
-    extern "C" __global__ __launch_bounds__(256) void hpmc_emit_##SUFFIX(                      \
-        hpmc::Geom g, float T, const void* fsrc, const uint2* items, const uint32_t* l4_off,   \
-        const hpmc::Totals* totals, float* out, hpmc::CellEdge* out_ce, uint32_t capacity,    \
-        const uint64_t* cases)                                                                \
-    {                                                                                         \
-        __shared__ hpmc::EmitShared sh;                                                       \
-        sh.cases[threadIdx.x] = cases[threadIdx.x];                                           \
-        sh.lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);                           \
-        __syncthreads();                                                                      \
-        hpmc::emit_body(MAKE_FIELD, g, T, items, l4_off, totals, out, out_ce, capacity, sh);  \
-    }                                                                                         \
     extern "C" __global__ __launch_bounds__(256) void hpmc_codes_##SUFFIX(                     \
         hpmc::Geom g, float T, const void* fsrc, uint8_t* codes)                              \
     {                                                                                         \
         hpmc::codes_body(MAKE_FIELD, g, T, codes);                                            \
     }
+
+#define HPMC_DEFINE_EMIT_KERNEL(NAME, MAKE_FIELD, MODE)                                        \
+    extern "C" __global__ __launch_bounds__(256) void NAME(                                   \
+        hpmc::Geom g, float T, const void* fsrc, const uint2* items, const uint32_t* l4_off,   \
+        const hpmc::Totals* totals, float* out, hpmc::CellEdge* out_ce, uint32_t capacity,    \
+        const uint64_t* cases, const float4* bnorm)                                           \
+    {                                                                                         \
+        __shared__ hpmc::EmitShared sh;                                                       \
+        sh.cases[threadIdx.x] = cases[threadIdx.x];                                           \
+        sh.lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);                           \
+        __syncthreads();                                                                      \
+        hpmc::emit_body<MODE>(MAKE_FIELD, g, T, items, l4_off, totals, out, out_ce, capacity, \
+                              bnorm, sh);                                                     \
+    }
+

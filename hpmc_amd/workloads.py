@@ -76,6 +76,38 @@ __device__ float HPMC_fetch(float3 p)
 """
 
 
+# The same field with its analytic gradient in p (hpmc.h:409-444 contract:
+# xyz = gradient, w = value): d/dp of f(2p - 1) = 2 f'(x).
+CAYLEY_GRAD_SNIPPET = CAYLEY_SNIPPET + r"""
+__device__ float4 HPMC_fetchGrad(float3 p)
+{
+    float x = 2.0f * p.x - 1.0f;
+    float y = 2.0f * p.y - 1.0f;
+    float z = 2.0f * p.z - 1.0f;
+    float gx = 16.0f * x + 32.0f * y * z;
+    float gy = 16.0f * y + 32.0f * x * z;
+    float gz = 16.0f * z + 32.0f * x * y;
+    return make_float4(gx, gy, gz, HPMC_fetch(p));
+}
+"""
+
+
+def cayley_grad(size, extra=0):
+    """(Z, Y, X, 4) float32: CAYLEY_GRAD_SNIPPET's HPMC_fetchGrad at every
+    lattice point, in the snippet's FP32 operation order."""
+    px, py, pz = _grid(size, extra)
+    f32 = np.float32
+    x = f32(2.0) * px - f32(1.0)
+    y = f32(2.0) * py - f32(1.0)
+    z = f32(2.0) * pz - f32(1.0)
+    out = np.zeros(np.broadcast_shapes(x.shape, y.shape, z.shape) + (4,), np.float32)
+    out[..., 0] = f32(16.0) * x + (f32(32.0) * y) * z
+    out[..., 1] = f32(16.0) * y + (f32(32.0) * x) * z
+    out[..., 2] = f32(16.0) * z + (f32(32.0) * x) * y
+    out[..., 3] = cayley(size, extra)
+    return out
+
+
 def cayley(size, extra=0):
     px, py, pz = _grid(size, extra)
     f32 = np.float32

@@ -197,9 +197,13 @@ int32_t HPMCsetProfiling(struct HPMCHistoPyramid* h, int32_t enable);
 int32_t HPMCgetStageTimes(struct HPMCHistoPyramid* h, double* ms, uint32_t* launches);
 
 /* Compiles a field snippet (as HPMCsetFieldCustom takes it) for gfx950
- * without touching a device; the compiler log goes to `log`.  Returns
- * HPMC_OK or HPMC_ERR_COMPILE. */
-int32_t HPMCcheckFieldSnippet(const char* source, char* log, size_t log_bytes);
+ * without touching a device; the compiler log goes to `log`.  `flags`:
+ * HPMC_SNIPPET_GRADIENT (the snippet also defines HPMC_fetchGrad) and/or
+ * HPMC_SNIPPET_BINARY (binary-field extraction).  Returns HPMC_OK or
+ * HPMC_ERR_COMPILE. */
+#define HPMC_SNIPPET_GRADIENT 1u
+#define HPMC_SNIPPET_BINARY 2u
+int32_t HPMCcheckFieldSnippet(const char* source, uint32_t flags, char* log, size_t log_bytes);
 
 /* Layout of the reference base level (hpmc/src/init.cpp:105-130):
  * out = {tile_x, tile_y, tiles_x, tiles_y, size_l2}. */

@@ -18,6 +18,12 @@ own formulas; the tests that do it are named in brackets):
                   (test_emit_*)
   cases           fixture from hpmc/src/tables.cpp plus per-case topology
                   invariants (tests/test_mc_table.py)
+  binary_normals  the emitted normal equals |n| n for the face normal n of
+                  the triangle through the emitted midpoints, up to the
+                  table's clamp and FP32 rounding (test_binary_*)
+  emit_binary     midpoints are exact; normals as above
+  emit_gradient   a linear field with its exact gradient gives exact t,
+                  positions and normals T - grad (test_emit_gradient_*)
 """
 import ctypes
 import json
@@ -75,6 +81,10 @@ def lib():
         L.hpo_emit_f32.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, ctypes.c_float, vp, vp, vp, i64, vp]
         L.hpo_run_volume.restype = i64
         L.hpo_run_volume.argtypes = [vp, vp, vp, vp, ctypes.c_float, ctypes.c_int32, ctypes.c_int32, vp, i64]
+        L.hpo_binary_normals.argtypes = [vp]
+        L.hpo_emit_binary_f64.argtypes = [vp, vp, vp, vp, vp, i64, vp]
+        L.hpo_emit_grad_f64.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, ctypes.c_float, vp, vp, vp, i64, vp, vp,
+                                        vp]
         L.hpo_morton2.restype = ctypes.c_uint64
         L.hpo_morton2.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         counts, slots = load_cases()
@@ -158,6 +168,46 @@ def emit(field, codes, cell, slot, T, ext=(1.0, 1.0, 1.0), cells=None, size=None
     lib().hpo_emit_f32(_ptr(field), _ptr(dims), _ptr(size), int(clamp), _ptr(cells), _ptr(ext), float(T),
                        _ptr(codes), _ptr(cell), _ptr(slot), n, _ptr(out))
     return out[:n]
+
+
+def binary_normals():
+    """(256, 16, 3) float32: the normal a binary field's vertex carries, by
+    HPMC code and slot (constants.cpp:217-273 decoded as shadergen.cpp:507-510)."""
+    out = np.zeros((256, 16, 3), np.float32)
+    lib().hpo_binary_normals(_ptr(out))
+    return out
+
+
+def emit_binary(codes, cell, slot, cells, ext=(1.0, 1.0, 1.0)):
+    """N3F_V3F (N, 6) float64 of a binary field: edge midpoints and tabulated normals."""
+    codes = np.ascontiguousarray(codes, np.uint8)
+    cell = np.ascontiguousarray(cell, np.uint32)
+    slot = np.ascontiguousarray(slot, np.uint8)
+    n = len(cell)
+    out = np.zeros((max(n, 1), 6), np.float64)
+    lib().hpo_emit_binary_f64(_ptr(_i32(cells)), _ptr(np.ascontiguousarray(ext, np.float32)), _ptr(codes),
+                              _ptr(cell), _ptr(slot), n, _ptr(out))
+    return out[:n]
+
+
+def emit_gradient(field4, codes, cell, slot, T, ext=(1.0, 1.0, 1.0), cells=None, size=None, clamp=True):
+    """Gradient-field emission: field4 is (fz, fy, fx, 4) float32 (gradient,
+    value); returns (out (N, 6) float64, t, mag)."""
+    field4 = np.ascontiguousarray(field4, np.float32)
+    dims = np.array(field4.shape[:3][::-1], np.int32)
+    size = _i32(size if size is not None else dims)
+    cells = _i32(cells if cells is not None else size - 1)
+    codes = np.ascontiguousarray(codes, np.uint8)
+    cell = np.ascontiguousarray(cell, np.uint32)
+    slot = np.ascontiguousarray(slot, np.uint8)
+    n = len(cell)
+    out = np.zeros((max(n, 1), 6), np.float64)
+    t = np.zeros(max(n, 1), np.float64)
+    mag = np.zeros(max(n, 1), np.float64)
+    lib().hpo_emit_grad_f64(_ptr(field4), _ptr(dims), _ptr(size), int(clamp), _ptr(cells),
+                            _ptr(np.ascontiguousarray(ext, np.float32)), float(T), _ptr(codes), _ptr(cell),
+                            _ptr(slot), n, _ptr(out), _ptr(t), _ptr(mag))
+    return out[:n], t[:n], mag[:n]
 
 
 def run_volume(field, T, ext=(1.0, 1.0, 1.0), z_begin=0, z_end=None, emit_out=True):

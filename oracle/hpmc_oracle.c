@@ -353,6 +353,117 @@ void hpo_emit_f32(const float* f, const int32_t dims[3], const int32_t size[3], 
 }
 
 /* ------------------------------------------------------------------------ */
+/* Binary fields.  constants.cpp:217-273: for each case and triangle, the    */
+/* face normal n = (m2-m0) x (m1-m0) over the midpoints m of its three edges */
+/* (tables.cpp:314-323), scaled by s = 0.5|n|, offset by 0.5 and clamped to  */
+/* (0,1) (tmp < 0 -> 1e-6, tmp >= 1 -> 0.999999), is added to the integer    */
+/* shift of every slot of the triangle; shadergen.cpp:507-510 decodes it as  */
+/* 2 fract(shift + tmp) - 1.  FP32 in that order.  out = 256*16*3 floats by  */
+/* HPMC code and slot (zero past the case's count).                         */
+void hpo_binary_normals(float* out)
+{
+    memset(out, 0, sizeof(float) * 256 * 16 * 3);
+    for (int c = 0; c < 256; ++c) {
+        for (int i = 0; i < g_count[c]; ++i) {
+            int t = i / 3;
+            float m[3][3];
+            for (int k = 0; k < 3; ++k) {
+                const int* e = g_slot[c][3 * t + k];
+                for (int q = 0; q < 3; ++q) m[k][q] = (float)e[q] + (q == e[3] ? 0.5f : 0.0f);
+            }
+            float u[3], v[3], n[3];
+            for (int q = 0; q < 3; ++q) {
+                u[q] = m[2][q] - m[0][q];
+                v[q] = m[1][q] - m[0][q];
+            }
+            n[0] = u[1] * v[2] - u[2] * v[1];
+            n[1] = u[2] * v[0] - u[0] * v[2];
+            n[2] = u[0] * v[1] - u[1] * v[0];
+            float s = 0.5f * sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            const int* e = g_slot[c][i];
+            for (int q = 0; q < 3; ++q) {
+                float tmp = s * n[q] + 0.5f;
+                if (tmp < 0.f) tmp = 0.000001f;
+                if (tmp >= 1.f) tmp = 0.999999f;
+                float packed = (float)e[q] + tmp;
+                out[(c * 16 + i) * 3 + q] = 2.0f * (packed - floorf(packed)) - 1.0f;
+            }
+        }
+    }
+}
+
+/* Binary emission (shadergen.cpp:507-510, 521-522, 556-563): the vertex is */
+/* the edge midpoint, p = EXT*(a + 0.5*axis)/CELLS; the normal is the       */
+/* tabulated one times EXT/CELLS.  FP64 on the FP32 table.                  */
+void hpo_emit_binary_f64(const int32_t cells[3], const float ext[3], const uint8_t* codes,
+                         const uint32_t* cell, const uint8_t* slot, int64_t n, double* out6)
+{
+    static float table[256 * 16 * 3];
+    hpo_binary_normals(table);
+    double sc[3];
+    for (int a = 0; a < 3; ++a) sc[a] = (double)ext[a] / (double)cells[a];
+    for (int64_t v = 0; v < n; ++v) {
+        uint32_t c = cell[v];
+        int64_t x = c % cells[0], y = (c / cells[0]) % cells[1], z = c / ((int64_t)cells[0] * cells[1]);
+        const int* e = g_slot[codes[c]][slot[v]];
+        int64_t a[3] = {x + e[0], y + e[1], z + e[2]};
+        double* o = out6 + 6 * v;
+        for (int q = 0; q < 3; ++q) {
+            o[q] = (double)table[(codes[c] * 16 + slot[v]) * 3 + q] * sc[q];
+            o[3 + q] = ((double)a[q] + (q == e[3] ? 0.5 : 0.0)) * sc[q];
+        }
+    }
+}
+
+/* Gradient-providing fields (shadergen.cpp:539-549): g is an RGBA lattice  */
+/* (gradient xyz, value w; hpmc.h:116-149), classification uses w.  With fa, */
+/* fb the texels at the edge ends: t = (fa.w-T)/(fa.w-fb.w), p as in the    */
+/* continuous case, n = (T - mix(fa.xyz, fb.xyz, t)) * EXT/CELLS.  FP64 on  */
+/* the FP32 texels; out_mag as in hpo_emit_f64.                             */
+static inline const float* texel4(const float* g, const int32_t dims[3], const int32_t size[3], int clamp,
+                                  int64_t i, int64_t j, int64_t k)
+{
+    if (clamp) {
+        i = i < 0 ? 0 : (i >= size[0] ? size[0] - 1 : i);
+        j = j < 0 ? 0 : (j >= size[1] ? size[1] - 1 : j);
+        k = k < 0 ? 0 : (k >= size[2] ? size[2] - 1 : k);
+    }
+    return g + 4 * ((k * dims[1] + j) * dims[0] + i);
+}
+
+void hpo_emit_grad_f64(const float* g4, const int32_t dims[3], const int32_t size[3], int clamp,
+                       const int32_t cells[3], const float ext[3], float T,
+                       const uint8_t* codes, const uint32_t* cell, const uint8_t* slot, int64_t n,
+                       double* out6, double* out_t, double* out_mag)
+{
+    double sc[3];
+    for (int a = 0; a < 3; ++a) sc[a] = (double)ext[a] / (double)cells[a];
+    for (int64_t v = 0; v < n; ++v) {
+        uint32_t c = cell[v];
+        int64_t x = c % cells[0], y = (c / cells[0]) % cells[1], z = c / ((int64_t)cells[0] * cells[1]);
+        const int* e = g_slot[codes[c]][slot[v]];
+        int64_t a[3] = {x + e[0], y + e[1], z + e[2]};
+        int ax = e[3];
+        int64_t b[3] = {a[0], a[1], a[2]};
+        b[ax] += 1;
+        const float* fa = texel4(g4, dims, size, clamp, a[0], a[1], a[2]);
+        const float* fb = texel4(g4, dims, size, clamp, b[0], b[1], b[2]);
+        double t = ((double)fa[3] - (double)T) / ((double)fa[3] - (double)fb[3]);
+        double* o = out6 + 6 * v;
+        for (int q = 0; q < 3; ++q) {
+            o[q] = ((double)T - ((double)fa[q] * (1.0 - t) + (double)fb[q] * t)) * sc[q];
+            o[3 + q] = ((double)a[q] + (q == ax ? t : 0.0)) * sc[q];
+        }
+        if (out_t) out_t[v] = t;
+        if (out_mag) {
+            double m = fabs((double)T);
+            for (int q = 0; q < 3; ++q) m = fmax(m, fmax(fabs((double)fa[q]), fabs((double)fb[q])));
+            out_mag[v] = m;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Whole path on the CPU for a volume (the benchmark's CPU baseline):        */
 /* classify -> reference-order vertex list -> FP32 emission, restricted to   */
 /* cell slices [z_begin, z_end).  Returns the vertex count of those slices.  */

@@ -73,6 +73,14 @@ def test_null_handles_follow_reference_conventions():
 @pytest.mark.parametrize("src", [workloads.CAYLEY_SNIPPET, workloads.METABALL_SNIPPET])
 def test_snippets_compile_for_gfx950(src):
     hpmc_amd.check_snippet(src)
+    hpmc_amd.check_snippet(src, binary=True)
+
+
+def test_gradient_snippet_compiles_and_needs_fetch_grad():
+    hpmc_amd.check_snippet(workloads.CAYLEY_GRAD_SNIPPET, gradient=True)
+    hpmc_amd.check_snippet(workloads.CAYLEY_GRAD_SNIPPET, gradient=True, binary=True)
+    with pytest.raises(hpmc_amd.HPMCError, match="HPMC_fetchGrad"):
+        hpmc_amd.check_snippet(workloads.CAYLEY_SNIPPET, gradient=True)
 
 
 def test_bad_snippet_reports_compile_error():

@@ -1,0 +1,85 @@
+"""GPU parity for binary fields and gradient-providing fields (needs an
+MI355X).  Reference: shadergen.cpp:93-101 (binary threshold 0.5),
+507-510 and 521-522 (midpoints and tabulated normals), 539-549 (gradient
+fields); constants.cpp:217-273 (the binary normal table).  Comparison and
+tolerances: tests/parity_util.py."""
+import numpy as np
+import pytest
+
+from hpmc_amd import IsoSurface, workloads
+from parity_util import check, gpu_run, smooth_noise, with_gradient
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("lattice", [(33, 20, 45), (64, 64, 64)])
+def test_binary_volume(lattice):
+    # a 0/1 field as a binary texture would hold; the build's threshold is
+    # ignored in favour of 0.5
+    f = (smooth_noise(lattice, 21) > 0.5).astype(np.float32)
+    res = gpu_run(lattice, 0.123, volume=f, binary=True)
+    assert check(res, f, 0.123, lattice, mode="binary") > 0
+
+
+def test_binary_volume_non_binary_values_grid_and_extent():
+    lattice, grid, ext = (48, 40, 36), (30, 25, 33), (2.0, 0.5, 1.5)
+    f = smooth_noise(lattice, 4)
+    res = gpu_run(lattice, 7.0, volume=f, grid=grid, ext=ext, binary=True)
+    assert check(res, f, 7.0, lattice, grid=grid, ext=ext, mode="binary") > 0
+
+
+def test_binary_snippet_cayley():
+    lattice = (96, 96, 96)
+    res = gpu_run(lattice, 0.0, snippet=workloads.CAYLEY_SNIPPET, binary=True)
+    f = workloads.cayley(lattice, extra=2)
+    assert check(res, f, 0.0, lattice, clamp=False, mode="binary") > 1000
+
+
+def test_binary_flag_applies_at_reconfiguration():
+    # hpiface.cpp:131-142: the flag does not taint the HistoPyramid
+    lattice = (40, 40, 40)
+    f = smooth_noise(lattice, 8)
+    vol = torch.from_numpy(f).cuda()
+    iso = IsoSurface(lattice)
+    iso.set_volume(vol)
+    iso.build(0.3)
+    n_cont = iso.count()
+    iso.set_binary(True)
+    iso.build(0.3)
+    assert iso.count() == n_cont            # still continuous, T = 0.3
+    iso.set_volume(vol)                     # same mode: no reconfiguration
+    iso.build(0.3)
+    assert iso.count() == n_cont
+    from hpmc_amd import lib
+    lib().HPMCsetGridExtent(iso.h, 1.0, 1.0, 1.0)   # taints
+    iso.build(0.3)
+    import oracle
+    _, counts = oracle.classify(f, 0.5)
+    assert iso.count() == int(counts.sum())
+    iso.close()
+
+
+@pytest.mark.parametrize("lattice", [(33, 20, 45), (80, 72, 64)])
+def test_gradient_volume(lattice):
+    f = smooth_noise(lattice, 31)
+    f4 = with_gradient(f)
+    res = gpu_run(lattice, 0.5, volume=f4, gradient=True)
+    assert check(res, f, 0.5, lattice, mode="gradient", field4=f4) > 0
+
+
+def test_gradient_volume_binary():
+    # a gradient texture in binary mode: classify the value channel at 0.5,
+    # emit midpoints and tabulated normals
+    lattice = (40, 36, 30)
+    f = smooth_noise(lattice, 41)
+    res = gpu_run(lattice, 0.9, volume=with_gradient(f), gradient=True, binary=True)
+    assert check(res, f, 0.9, lattice, mode="binary") > 0
+
+
+def test_gradient_snippet_cayley():
+    lattice = (128, 128, 128)
+    res = gpu_run(lattice, 0.0, snippet=workloads.CAYLEY_GRAD_SNIPPET, gradient=True)
+    f4 = workloads.cayley_grad(lattice, extra=2)
+    f = np.ascontiguousarray(f4[..., 3])
+    assert check(res, f, 0.0, lattice, clamp=False, mode="gradient", field4=f4) > 10000

@@ -1,17 +1,8 @@
-"""GPU parity of the HIP path against the CPU oracle (needs an MI355X).
-
-Every case compares, on the same FP32 field:
-  * the HPMC corner code of every cell           -- bit-exact
-  * the level-4 pyramid sums (16x16-cell blocks) -- bit-exact
-  * the vertex count                             -- exact
-  * the (cell, edge) list in extraction order    -- bit-exact, in HPMC's order
-  * positions and normals against the oracle's FP64 evaluation on the same
-    FP32 samples, within
-        position: 1e-5 cell + 2^-22 (|k + t| + 1) cells  (1e-5 of the cell size
-                  plus the FP32 rounding of the absolute coordinate)
-        normal:   16 * 2^-24 * max(|T|, |f| of the samples used) * extent/cells
-                  (FP32 rounding of T - mix(na, nb, t))
-"""
+"""GPU parity of the HIP path against the CPU oracle (needs an MI355X),
+continuous fields: volumes and snippets.  The comparison and its tolerances
+are those of tests/parity_util.py (codes, level-4 sums, count and ordered
+(cell, edge) list bit-exact; positions and normals within FP32 bounds of the
+oracle's FP64 evaluation)."""
 import numpy as np
 import pytest
 
@@ -21,74 +12,7 @@ from hpmc_amd import IsoSurface, workloads
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
-EPS = 2.0 ** -24
-
-
-def _gpu(lattice, T, volume=None, snippet=None, uniforms=(), grid=None, ext=(1.0, 1.0, 1.0)):
-    iso = IsoSurface(lattice, grid=grid, extent=ext)
-    if volume is not None:
-        vol = torch.from_numpy(np.ascontiguousarray(volume)).cuda()
-        iso.set_volume(vol)
-    else:
-        iso.set_snippet(snippet)
-        for name, arr in uniforms:
-            iso.set_uniform(name, arr)
-    verts, ce = iso.extract_all(T, with_cell_edge=True)
-    codes = iso.cell_codes()
-    l4 = iso.level4()
-    torch.cuda.synchronize()
-    res = dict(n=iso.count(), verts=verts.cpu().numpy(), ce=ce.cpu().numpy().view(np.uint32),
-               codes=codes.cpu().numpy(), level4=l4.cpu().numpy().view(np.uint32), layout=iso.layout())
-    iso.close()
-    return res
-
-
-def _check(res, field, T, lattice, grid=None, ext=(1.0, 1.0, 1.0), clamp=True):
-    cells = grid or tuple(v - 1 for v in lattice)
-    codes, counts = oracle.classify(field, T, cells=cells, size=lattice)
-    assert np.array_equal(res["codes"], codes), "cell codes differ"
-    lay = oracle.layout(lattice, cells)
-    assert tuple(res["layout"]) == lay
-    l4 = oracle.level4(counts, cells, lay)
-    assert np.array_equal(res["level4"], l4), "level-4 sums differ"
-    cell, slot = oracle.vertex_list(counts, cells, lay)
-    assert res["n"] == len(cell)
-    assert np.array_equal(res["ce"][:, 0], cell), "vertex cells differ (order or content)"
-    assert np.array_equal(res["ce"][:, 1], oracle.edge_ids(codes, cell, slot)), "vertex edges differ"
-    ref, t, mag = oracle.emit(field, codes, cell, slot, T, ext=ext, cells=cells, size=lattice, clamp=clamp,
-                              with_mag=True)
-    got = res["verts"].astype(np.float64)
-    sc = np.asarray(ext, np.float64) / np.asarray(cells, np.float64)
-    pos_cells = ref[:, 3:] / sc
-    dpos = np.abs(got[:, 3:] - ref[:, 3:]) / sc
-    tol_pos = 1e-5 + 2.0 ** -22 * (np.abs(pos_cells) + 1.0)
-    assert (dpos <= tol_pos).all(), "position error %g cells" % float((dpos - tol_pos).max())
-    dn = np.abs(got[:, :3] - ref[:, :3])
-    tol_n = 16 * EPS * mag[:, None] * sc[None, :]
-    assert (dn <= tol_n).all(), "normal error exceeds FP32 bound by %g" % float((dn - tol_n).max())
-    assert ((t >= 0) & (t <= 1)).all()
-    return len(cell)
-
-
-def _smooth_noise(shape, seed):
-    """Random field with structure at a few cells (upsampled coarse noise)."""
-    X, Y, Z = shape
-    rng = np.random.default_rng(seed)
-    coarse = rng.random((Z // 4 + 2, Y // 4 + 2, X // 4 + 2))
-    zi = np.arange(Z) / 4.0
-    yi = np.arange(Y) / 4.0
-    xi = np.arange(X) / 4.0
-
-    def interp(a, coords, axis):
-        i0 = np.floor(coords).astype(int)
-        w = coords - i0
-        a0 = np.take(a, i0, axis=axis)
-        a1 = np.take(a, i0 + 1, axis=axis)
-        shp = [1, 1, 1]
-        shp[axis] = len(coords)
-        return a0 * (1 - w.reshape(shp)) + a1 * w.reshape(shp)
-
-    return interp(interp(interp(coarse, xi, 2), yi, 1), zi, 0).astype(np.float32)
+from parity_util import EPS, gpu_run as _gpu, check as _check, smooth_noise as _smooth_noise  # noqa: E402
 
 
 @pytest.mark.parametrize("lattice", [(17, 17, 17), (33, 20, 45), (64, 64, 64), (100, 37, 29), (130, 70, 9),

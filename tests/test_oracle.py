@@ -196,3 +196,90 @@ def test_run_volume_counts():
     f = workloads.cayley((40, 40, 40))
     codes, counts = oracle.classify(f, 0.0)
     assert oracle.run_volume(f, 0.0) == counts.sum()
+
+
+# --------------------------------------------------------- binary fields ---
+def _midpoint(s):
+    m = np.array(s[:3], np.float64)
+    m[s[3]] += 0.5
+    return m
+
+
+def test_binary_normal_known_answer():
+    # code 1: only corner (0,0,0) is inside; one triangle through the three
+    # midpoints next to it, face normal of length sqrt(3)/4 along the body
+    # diagonal, emitted as |n| n = -(sqrt(3)/16) (1,1,1), toward the corner
+    bn = oracle.binary_normals()
+    assert oracle.CASE_COUNTS[1] == 3
+    np.testing.assert_allclose(bn[1, :3], -np.sqrt(3.0) / 16.0, rtol=0, atol=2e-7)
+    # its complement: every corner but (0,0,0) inside, the normal flips
+    np.testing.assert_allclose(bn[254, :3], np.sqrt(3.0) / 16.0, rtol=0, atol=2e-7)
+
+
+def test_binary_normals_are_scaled_face_normals_toward_inside():
+    """For all 256 cases: each slot's normal is |nf| nf for the face normal nf
+    of its triangle (geometry of the edge midpoints) oriented toward the
+    inside corners (f < T), like the continuous forward-difference normals;
+    components with |nf| |nf_k| >= 1 saturate at the table's clamp."""
+    bn = oracle.binary_normals().astype(np.float64)
+    checked = 0
+    for code in range(256):
+        inside = lambda p: (code >> (int(p[0]) + 2 * int(p[1]) + 4 * int(p[2]))) & 1
+        for t in range(oracle.CASE_COUNTS[code] // 3):
+            tri = oracle.CASE_SLOTS[code, 3 * t:3 * t + 3]
+            m = [_midpoint(s) for s in tri]
+            nf = np.cross(m[1] - m[0], m[2] - m[0])
+            score = 0.0
+            for s in tri:
+                a = np.array(s[:3], np.float64)
+                b = a.copy()
+                b[s[3]] += 1
+                if not inside(a):
+                    a, b = b, a
+                score += float(np.dot(nf, a - b))
+            assert abs(score) > 1e-9
+            if score < 0:
+                nf = -nf
+            want = np.linalg.norm(nf) * nf
+            sat = np.abs(want) >= 1.0
+            want_c = np.where(sat, np.sign(want) * (1.0 - 2e-6), want)
+            for i in range(3):
+                got = bn[code, 3 * t + i]
+                assert (np.abs(got - want_c) <= np.where(sat, 3e-6, 3e-7)).all(), (code, t, i, got, want_c)
+                checked += 1
+    assert checked == int(oracle.CASE_COUNTS.sum())
+
+
+def test_emit_binary_midpoints():
+    f = (workloads.cayley((21, 19, 17)) < 0.25).astype(np.float32)
+    codes, cell, slot, cells = _list(f, 0.5)
+    out = oracle.emit_binary(codes, cell, slot, cells, ext=(2.0, 1.0, 0.5))
+    assert len(out) > 0
+    g = out[:, 3:] / (np.array([2.0, 1.0, 0.5]) / np.array(cells))
+    # every coordinate is a multiple of half a cell, and exactly one (the
+    # edge axis) sits halfway between lattice points
+    h = np.round(2.0 * g)
+    assert (np.abs(2.0 * g - h) < 1e-9).all()
+    assert ((h.astype(np.int64) % 2).sum(axis=1) == 1).all()
+
+
+# ------------------------------------------------------- gradient fields ---
+def test_emit_gradient_linear_field_exact():
+    # f = 0.5 x - 0.25 y + 0.125 z (lattice units) with its exact gradient:
+    # t, positions and normals are exact, n = (T - grad) * ext/cells
+    X, Y, Z = 11, 9, 7
+    k = np.arange(Z)[:, None, None]
+    j = np.arange(Y)[None, :, None]
+    i = np.arange(X)[None, None, :]
+    val = (0.5 * i - 0.25 * j + 0.125 * k).astype(np.float32)
+    f4 = np.zeros((Z, Y, X, 4), np.float32)
+    f4[..., 0], f4[..., 1], f4[..., 2], f4[..., 3] = 0.5, -0.25, 0.125, val
+    T = 1.3
+    codes, cell, slot, cells = _list(val, T)
+    ext = (1.0, 2.0, 3.0)
+    out, t, _ = oracle.emit_gradient(f4, codes, cell, slot, T, ext=ext)
+    assert len(out) > 0 and ((t >= 0) & (t <= 1)).all()
+    sc = np.array(ext) / np.array(cells)
+    p = out[:, 3:] / sc
+    np.testing.assert_allclose(0.5 * p[:, 0] - 0.25 * p[:, 1] + 0.125 * p[:, 2], np.float32(T), atol=1e-6)
+    np.testing.assert_allclose(out[:, :3], np.broadcast_to((np.float32(T) - np.array([0.5, -0.25, 0.125])) * sc, out[:, :3].shape), atol=1e-12)
