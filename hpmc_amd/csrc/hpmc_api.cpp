@@ -27,36 +27,42 @@
 // Device sources embedded for hipRTC (generated by hpmc_amd/build.py).
 #include "_embedded_sources.inc"
 
-#define HPMC_VOLUME_FIELD (hpmc::VolumeField{(const float*)fsrc, g.sx, g.sy, g.sz})
-#define HPMC_VOLUME4_FIELD (hpmc::Volume4Field{(const float4*)fsrc, g.sx, g.sy, g.sz})
+// Built-in kernels for FP32 volume fields (scalar, and RGBA = gradient +
+// value); snippet fields get the same kernels from hipRTC.
+#define HPMC_VOLUME_FIELD (hpmc::VolumeField{(const float*)dd.field, dd.g.sx, dd.g.sy, dd.g.sz})
+#define HPMC_VOLUME4_FIELD (hpmc::Volume4Field{(const float4*)dd.field, dd.g.sx, dd.g.sy, dd.g.sz})
 HPMC_DEFINE_BUILD_KERNELS(volume, HPMC_VOLUME_FIELD)
 HPMC_DEFINE_BUILD_KERNELS(volume4, HPMC_VOLUME4_FIELD)
-HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume, HPMC_VOLUME_FIELD, hpmc::kEmitContinuous)
-HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume_binary, HPMC_VOLUME_FIELD, hpmc::kEmitBinary)
-HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume4_gradient, HPMC_VOLUME4_FIELD, hpmc::kEmitGradient)
-HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume4_binary, HPMC_VOLUME4_FIELD, hpmc::kEmitBinary)
+HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume, HPMC_VOLUME_FIELD, hpmc::kEmitContinuous, false)
+HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume_binary, HPMC_VOLUME_FIELD, hpmc::kEmitBinary, false)
+HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume4_gradient, HPMC_VOLUME4_FIELD, hpmc::kEmitGradient, false)
+HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume4_binary, HPMC_VOLUME4_FIELD, hpmc::kEmitBinary, false)
+HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume_batch, HPMC_VOLUME_FIELD, hpmc::kEmitContinuous, true)
 
-extern "C" __global__ __launch_bounds__(256) void hpmc_scan_reduce(hpmc::Geom g, uint32_t n, uint32_t ntiles,
-                                                                   const uint16_t* colcnt, uint32_t ncc,
+extern "C" __global__ __launch_bounds__(256) void hpmc_scan_reduce(const hpmc::VolDesc* vols, const uint32_t* tile_vol,
+                                                                   uint32_t n_l4t, const uint16_t* colcnt,
                                                                    uint2* tile_sums)
 {
-    hpmc::scan_reduce_body(g, n, ntiles, colcnt, ncc, tile_sums);
+    hpmc::scan_reduce_body(vols, tile_vol, n_l4t, colcnt, tile_sums);
 }
 
-extern "C" __global__ __launch_bounds__(256) void hpmc_scan_scatter(hpmc::Geom g, uint32_t n, uint32_t ntiles,
-                                                                    const uint16_t* colcnt, uint32_t ncc,
-                                                                    const uint2* tile_sums, uint32_t nctiles,
-                                                                    uint32_t* l4_off, uint2* items,
-                                                                    hpmc::Totals* totals, hpmc::Totals* host_totals)
+extern "C" __global__ __launch_bounds__(256) void hpmc_scan_scatter(
+    const hpmc::VolDesc* vols, const uint32_t* tile_vol, uint32_t n_l4t, uint32_t n_colt, const uint16_t* colcnt,
+    const uint2* tile_sums, uint32_t* l4_off, uint2* items, hpmc::Totals* totals, hpmc::Totals* host_totals,
+    hpmc::VolCount* vol_counts, hpmc::VolCount* host_vol_counts)
 {
-    hpmc::scan_scatter_body(g, n, ntiles, colcnt, ncc, tile_sums, nctiles, l4_off, items, totals, host_totals);
+    hpmc::scan_scatter_body(vols, tile_vol, n_l4t, n_colt, colcnt, tile_sums, l4_off, items, totals, host_totals,
+                            vol_counts, host_vol_counts);
 }
 
-extern "C" __global__ __launch_bounds__(256) void hpmc_level4_gather(hpmc::Geom g, const uint16_t* colcnt,
-                                                                     uint32_t n, uint32_t* out)
+extern "C" __global__ __launch_bounds__(256) void hpmc_level4_gather(const hpmc::VolDesc* vols, uint32_t vol,
+                                                                     const uint16_t* colcnt, uint32_t* out)
 {
-    hpmc::level4_body(g, colcnt, n, out);
+    const hpmc::VolDesc d = vols[vol];
+    hpmc::level4_body(d, colcnt, out);
 }
+
+#include "hpmc_plan.h"
 
 namespace {
 
@@ -146,22 +152,10 @@ struct HPMCHistoPyramid {
     int32_t tile[2] = {0, 0};
     int32_t tiles[2] = {0, 0};
     int32_t l2 = 0;
-    hpmc::Geom geom{};
-    int zseg = 16;
-    // device state
-    uint64_t n_level4 = 0;           // level-4 blocks in Morton key order (incl. blocks outside the grid)
-    uint2* tile_sums = nullptr;
-    uint32_t n_tiles = 0;
-    uint32_t* l4_off = nullptr;
-    uint16_t* colcnt = nullptr;      // block vertex counts in (z segment, y, x, z) order
-    uint32_t n_cc = 0;
-    uint32_t n_ctiles = 0;
-    uint2* items = nullptr;          // emit work items: block key, first vertex | count << 16
-    uint64_t n_items_max = 0;
-    int emit_grid = 0;
-    hpmc::Totals* d_totals = nullptr;
-    hpmc::Totals* h_totals = nullptr;  // pinned
-    hipEvent_t ev_totals = nullptr;
+    // device state of the build (one volume)
+    hpmc_host::Plan plan;
+    hpmc_host::Kernel k_classify, k_emit, k_codes;
+    hipEvent_t ev_totals = nullptr;  // after the scan: totals are in pinned memory
     bool built = false;
     bool count_valid = false;
     uint32_t count = 0;
@@ -169,11 +163,6 @@ struct HPMCHistoPyramid {
     hipStream_t stream = nullptr;
     // hipRTC module for snippets
     hipModule_t module = nullptr;
-    hipFunction_t k_classify = nullptr, k_emit = nullptr, k_codes = nullptr;
-    // built-in kernels for volume fields, chosen at setup
-    const void* f_classify = nullptr;
-    const void* f_emit = nullptr;
-    const void* f_codes = nullptr;
     uint32_t builder_id = 0;
     // optional per-stage timing: event pairs recorded around each launch
     bool profiling = false;
@@ -231,62 +220,31 @@ void ok(HPMCHistoPyramid* h)
 
 void free_device(HPMCHistoPyramid* h)
 {
-    if (h->tile_sums) (void)hipFree(h->tile_sums);
-    if (h->l4_off) (void)hipFree(h->l4_off);
-    if (h->colcnt) (void)hipFree(h->colcnt);
-    if (h->items) (void)hipFree(h->items);
-    if (h->d_totals) (void)hipFree(h->d_totals);
-    if (h->h_totals) (void)hipHostFree(h->h_totals);
+    hpmc_host::plan_free(h->plan);
     if (h->ev_totals) (void)hipEventDestroy(h->ev_totals);
     if (h->module) (void)hipModuleUnload(h->module);
-    h->tile_sums = nullptr;
-    h->l4_off = nullptr;
-    h->colcnt = nullptr;
-    h->items = nullptr;
-    h->d_totals = nullptr;
-    h->h_totals = nullptr;
     h->ev_totals = nullptr;
     h->module = nullptr;
-    h->k_classify = h->k_emit = h->k_codes = nullptr;
+    h->k_classify = h->k_emit = h->k_codes = hpmc_host::Kernel();
     h->built = false;
     h->count_valid = false;
 }
 
-// hpmc/src/init.cpp:72-149.  FP32 log2f/ceilf/sqrtf as in the reference.
+// hpmc/src/init.cpp:72-149 (hpmc_host::layout_of).
 bool determine_layout(HPMCHistoPyramid* h)
 {
-    for (int a = 0; a < 3; ++a) {
-        if (h->size[a] < 2 || h->size[a] > 16384) {
-            fail(h, HPMC_ERR_SIZE, "lattice size %d out of [2,16384] along axis %d", h->size[a], a);
-            return false;
-        }
-        if (h->size[a] <= h->cells[a] || h->cells[a] < 1) {
-            fail(h, HPMC_ERR_SIZE, "grid size %d must be in [1, lattice-1=%d] along axis %d", h->cells[a],
-                 h->size[a] - 1, a);
-            return false;
-        }
-    }
-    // cells == 1 along x or y gives log2f(0.5f) = -1 and a negative shift in
-    // the reference (init.cpp:106-111); it cannot lay out such a grid either.
-    if (h->cells[0] < 2 || h->cells[1] < 2) {
-        fail(h, HPMC_ERR_SIZE, "grid must have at least 2 cells along x and y");
+    hpmc_host::Layout lay;
+    std::string err;
+    const int32_t rc = hpmc_host::layout_of(h->size, h->cells, &lay, &err);
+    if (rc != HPMC_OK) {
+        fail(h, rc, "%s", err.c_str());
         return false;
     }
-    int32_t ts0 = 1 << (int32_t)ceilf(log2f((float)h->cells[0] / 2.0f));
-    int32_t ts1 = 1 << (int32_t)ceilf(log2f((float)h->cells[1] / 2.0f));
-    float aspect = (float)ts0 / (float)ts1;
-    float l = ceilf(log2f(sqrtf((float)h->cells[2] / aspect)));
-    if (l < 0.0f) l = 0.0f;
-    int32_t lay0 = 1 << (int32_t)l;
-    int32_t lay1 = (h->cells[2] + lay0 - 1) / lay0;
-    int64_t w0 = (int64_t)ts0 * lay0, w1 = (int64_t)ts1 * lay1;
-    int32_t l2 = (int32_t)ceilf(log2f((float)(w0 > w1 ? w0 : w1)));
-    int32_t S = 1 << l2;
-    h->tile[0] = ts0;
-    h->tile[1] = ts1;
-    h->tiles[0] = S / ts0;
-    h->tiles[1] = S / ts1;
-    h->l2 = l2;
+    h->tile[0] = lay.tile[0];
+    h->tile[1] = lay.tile[1];
+    h->tiles[0] = lay.tiles[0];
+    h->tiles[1] = lay.tiles[1];
+    h->l2 = lay.l2;
     return true;
 }
 
@@ -330,10 +288,10 @@ std::string rtc_source(const std::string& snippet, bool gradient, bool binary)
          "#endif\n"
          "};\n"
          "}\n"
-         "HPMC_DEFINE_BUILD_KERNELS(rtc, (hpmc::SnippetField{g.ipx, g.ipy, g.ipz}))\n";
+         "HPMC_DEFINE_BUILD_KERNELS(rtc, (hpmc::SnippetField{dd.g.ipx, dd.g.ipy, dd.g.ipz}))\n";
     const char* mode = binary ? "hpmc::kEmitBinary" : gradient ? "hpmc::kEmitGradient" : "hpmc::kEmitContinuous";
-    s += std::string("HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_rtc, (hpmc::SnippetField{g.ipx, g.ipy, g.ipz}), ") + mode +
-         ")\n";
+    s += std::string("HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_rtc, (hpmc::SnippetField{dd.g.ipx, dd.g.ipy, dd.g.ipz}), ") +
+         mode + ", false)\n";
     return s;
 }
 
@@ -383,55 +341,36 @@ bool compile_snippet(HPMCHistoPyramid* h)
         return false;
     }
     HIP_TRY(h, hipModuleLoadData(&h->module, code.data()));
-    HIP_TRY(h, hipModuleGetFunction(&h->k_classify, h->module, "hpmc_classify_rtc"));
-    HIP_TRY(h, hipModuleGetFunction(&h->k_emit, h->module, "hpmc_emit_rtc"));
-    HIP_TRY(h, hipModuleGetFunction(&h->k_codes, h->module, "hpmc_codes_rtc"));
+    HIP_TRY(h, hipModuleGetFunction(&h->k_classify.rtc, h->module, "hpmc_classify_rtc"));
+    HIP_TRY(h, hipModuleGetFunction(&h->k_emit.rtc, h->module, "hpmc_emit_rtc"));
+    HIP_TRY(h, hipModuleGetFunction(&h->k_codes.rtc, h->module, "hpmc_codes_rtc"));
     return true;
 }
 
-// Classification launch shape: one workgroup of ceil(X/64) waves (at most
-// 16) per band of 16 cell rows and zseg slices.  zseg is picked so the
-// workgroups fill the device about once (each extra segment re-reads one
-// plane): the power of two in 8..32 nearest above that, 32 being the slots
-// of a block column in the count array.
-int classify_waves(const hpmc::Geom& g)
+// The device's share of the classification and emit launches (resident
+// workgroups per CU), probed once per process.
+const hpmc_host::DeviceShape& device_shape()
 {
-    int nw = (g.sx + 63) / 64;
-    if (nw > hpmc::kClassifyMaxWaves) nw = hpmc::kClassifyMaxWaves;
-    return nw;
-}
-
-int choose_zseg(const hpmc::Geom& g, int resident_wgs)
-{
-    const int nw = classify_waves(g);
-    const int64_t wgx = (g.sx + 64 * nw - 1) / (64 * nw);
-    const int64_t bands = wgx * ((g.cy + hpmc::kBlock - 1) / hpmc::kBlock);
-    int64_t target = resident_wgs > 0 ? resident_wgs : 1024;
-    int64_t z = (g.cz * bands + target - 1) / target;
-    int zs = 8;
-    while (zs < z && zs < hpmc::kColStride) zs *= 2;
-    return zs;
-}
-
-int emit_grid(const HPMCHistoPyramid* h)
-{
-    // persistent; a multiple of 8, never many more waves than work items can be
-    const uint64_t max_wg = ((h->n_items_max + hpmc::kEmitWaves - 1) / hpmc::kEmitWaves + 7) / 8 * 8;
-    return (int)std::min<uint64_t>((uint64_t)h->emit_grid, max_wg);
+    static hpmc_host::DeviceShape shape;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return;
+        (void)hipDeviceGetAttribute(&shape.cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&shape.emit_per_cu, (const void*)hpmc_emit_volume, 256, 0);
+    });
+    return shape;
 }
 
 // hpmc/src/init.cpp:48-69: layout, storage and builder programs, once per
-// configuration.
+// configuration.  Classification workgroups span whole lattice rows
+// (ceil(X/64) waves, at most 16) over zseg slices, zseg chosen so the
+// workgroups fill the device about once.
 bool setup(HPMCHistoPyramid* h)
 {
     if (!h->tainted) return true;
     free_device(h);
     if (!determine_layout(h)) return false;
-    const int32_t tsx2 = 2 * h->tile[0], tsy2 = 2 * h->tile[1];
-    if (tsx2 < hpmc::kBlock || tsy2 < hpmc::kBlock) {
-        fail(h, HPMC_ERR_UNSUPPORTED, "grids with 8 or fewer cells along x or y are not supported");
-        return false;
-    }
     if (h->mode == kFieldVolume && !h->volume) {
         fail(h, HPMC_ERR_FIELD, "no field volume set");
         return false;
@@ -445,73 +384,32 @@ bool setup(HPMCHistoPyramid* h)
     h->binary_active = h->binary;
     if (h->mode == kFieldVolume) {
         if (h->gradient) {
-            h->f_classify = (const void*)hpmc_classify_volume4;
-            h->f_codes = (const void*)hpmc_codes_volume4;
-            h->f_emit = h->binary_active ? (const void*)hpmc_emit_volume4_binary
-                                         : (const void*)hpmc_emit_volume4_gradient;
+            h->k_classify.builtin = (const void*)hpmc_classify_volume4;
+            h->k_codes.builtin = (const void*)hpmc_codes_volume4;
+            h->k_emit.builtin = h->binary_active ? (const void*)hpmc_emit_volume4_binary
+                                                 : (const void*)hpmc_emit_volume4_gradient;
         } else {
-            h->f_classify = (const void*)hpmc_classify_volume;
-            h->f_codes = (const void*)hpmc_codes_volume;
-            h->f_emit = h->binary_active ? (const void*)hpmc_emit_volume_binary : (const void*)hpmc_emit_volume;
+            h->k_classify.builtin = (const void*)hpmc_classify_volume;
+            h->k_codes.builtin = (const void*)hpmc_codes_volume;
+            h->k_emit.builtin = h->binary_active ? (const void*)hpmc_emit_volume_binary
+                                                 : (const void*)hpmc_emit_volume;
         }
     }
-    hpmc::Geom& g = h->geom;
-    g.sx = h->size[0];
-    g.sy = h->size[1];
-    g.sz = h->size[2];
-    g.cx = h->cells[0];
-    g.cy = h->cells[1];
-    g.cz = h->cells[2];
-    g.tsx2 = tsx2;
-    g.tsy2 = tsy2;
-    g.tiles_x = h->tiles[0];
-    g.l2 = h->l2;
-    g.blocks_log2 = h->l2 + 1 - hpmc::kBlockLog2;
-    g.scx = h->extent[0] / (float)h->cells[0];
-    g.scy = h->extent[1] / (float)h->cells[1];
-    g.scz = h->extent[2] / (float)h->cells[2];
-    g.ipx = 1.0f / (float)h->size[0];
-    g.ipy = 1.0f / (float)h->size[1];
-    g.ipz = 1.0f / (float)h->size[2];
-    {
-        int dev = 0, cus = 0, per_cu = 0;
-        HIP_TRY(h, hipGetDevice(&dev));
-        HIP_TRY(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)hpmc_classify_volume,
-                                                                 64 * classify_waves(g), 0));
-        h->zseg = choose_zseg(g, cus * (per_cu > 0 ? per_cu : 1));
+    hpmc_host::VolumeSpec spec;
+    for (int a = 0; a < 3; ++a) {
+        spec.size[a] = h->size[a];
+        spec.cells[a] = h->cells[a];
+        spec.extent[a] = h->extent[a];
     }
-    const uint64_t side = (uint64_t)1 << g.blocks_log2;
-    h->n_level4 = side * side;
-    h->n_tiles = (uint32_t)((h->n_level4 + hpmc::kScanTile - 1) / hpmc::kScanTile);
-    g.zseg = h->zseg;
-    g.nbx = (g.cx + hpmc::kBlock - 1) / hpmc::kBlock;
-    g.nby = (g.cy + hpmc::kBlock - 1) / hpmc::kBlock;
-    g.npz = (g.cz + h->zseg - 1) / h->zseg;
-    h->n_cc = (uint32_t)((uint64_t)g.nbx * g.nby * g.npz * hpmc::kColStride);
-    h->n_ctiles = (h->n_cc + hpmc::kScanTile - 1) / hpmc::kScanTile;
-    // a block holds at most 256 * 15 vertices: 15 work items
-    h->n_items_max = (uint64_t)g.nbx * g.nby * g.cz * 15;
-    HIP_TRY(h, hipMalloc(&h->colcnt, (size_t)h->n_cc * sizeof(uint16_t)));
-    HIP_TRY(h, hipMemset(h->colcnt, 0, (size_t)h->n_cc * sizeof(uint16_t)));
-    HIP_TRY(h, hipMalloc(&h->items, (size_t)h->n_items_max * sizeof(uint2)));
-    HIP_TRY(h, hipMalloc(&h->tile_sums, (h->n_tiles + h->n_ctiles) * sizeof(uint2)));
-    HIP_TRY(h, hipMalloc(&h->l4_off, h->n_level4 * sizeof(uint32_t)));
-    {
-        // persistent emit grid: as many 4-wave workgroups as fit at once
-        int dev = 0, cus = 0, per_cu = 0;
-        HIP_TRY(h, hipGetDevice(&dev));
-        HIP_TRY(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)hpmc_emit_volume, 256, 0));
-        // a multiple of 8 for the XCD-aware item mapping
-        h->emit_grid = ((cus > 0 ? cus : 256) * (per_cu > 0 ? per_cu : 4) + 7) / 8 * 8;
+    spec.field = h->mode == kFieldVolume ? (const void*)h->volume : nullptr;
+    std::string err;
+    const int32_t rc = hpmc_host::plan_create(h->plan, {spec}, hpmc_host::Tuning(),
+                                              (const void*)hpmc_classify_volume, device_shape(), &err);
+    if (rc != HPMC_OK) {
+        fail(h, rc, "%s", err.c_str());
+        return false;
     }
-    HIP_TRY(h, hipMalloc(&h->d_totals, sizeof(hpmc::Totals)));
-    HIP_TRY(h, hipMemset(h->d_totals, 0, sizeof(hpmc::Totals)));
-    HIP_TRY(h, hipHostMalloc(&h->h_totals, sizeof(hpmc::Totals), hipHostMallocDefault));
-    memset(h->h_totals, 0, sizeof(hpmc::Totals));
     HIP_TRY(h, hipEventCreateWithFlags(&h->ev_totals, hipEventDisableTiming));
-    HIP_TRY(h, hipDeviceSynchronize());
     if (h->mode == kFieldCustom) {
         if (!compile_snippet(h)) return false;
     }
@@ -531,54 +429,21 @@ void stage_mark(HPMCHistoPyramid* h, int stage)
     h->stage_events[stage].push_back(e);
 }
 
-const void* field_source(HPMCHistoPyramid* h) { return h->mode == kFieldVolume ? (const void*)h->volume : nullptr; }
-
-// Launches a field kernel: the built-in one for volume fields, the hipRTC
-// one for snippets; `args` point at the kernel's arguments in order.
-bool launch_field_kernel(HPMCHistoPyramid* h, const void* builtin, hipFunction_t rtc, unsigned grid,
-                         unsigned threads, void** args)
-{
-    if (h->mode == kFieldVolume) {
-        HIP_TRY(h, hipLaunchKernel(builtin, dim3(grid), dim3(threads), args, 0, h->stream));
-    } else {
-        HIP_TRY(h, hipModuleLaunchKernel(rtc, grid, 1, 1, threads, 1, 1, 0, h->stream, args, nullptr));
-    }
-    HIP_TRY(h, hipGetLastError());
-    return true;
-}
-
 bool launch_classify(HPMCHistoPyramid* h, float T)
 {
-    const hpmc::Geom& g = h->geom;
-    const int nw = classify_waves(g);
-    const int64_t wgx = (g.sx + 64 * nw - 1) / (64 * nw);
-    // the XCD-aware tile mapping needs the grid padded to a multiple of 8
-    const int64_t tiles = wgx * ((g.cy + hpmc::kBlock - 1) / hpmc::kBlock) * ((g.cz + h->zseg - 1) / h->zseg);
-    const unsigned blocks = (unsigned)(((tiles + 7) / 8) * 8);
-    const unsigned threads = 64u * (unsigned)nw;
-    hpmc::Geom ga = g;
-    float Ta = T;
-    const void* fa = field_source(h);
-    uint16_t* aw = h->colcnt;
-    int zs = h->zseg;
-    const uint64_t* cs = h->constants->d_cases;
-    void* args[] = {&ga, &Ta, &fa, &aw, &zs, &cs};
+    if (h->mode == kFieldVolume) HIP_TRY(h, hpmc_host::plan_set_field(h->plan, 0, h->volume, h->stream));
     stage_mark(h, kStageClassify);
-    if (!launch_field_kernel(h, h->f_classify, h->k_classify, blocks, threads, args)) return false;
+    HIP_TRY(h, hpmc_host::plan_classify(h->plan, h->k_classify, T, h->constants->d_cases, h->stream));
+    HIP_TRY(h, hipGetLastError());
     stage_mark(h, kStageClassify);
     return true;
 }
 
 bool launch_scan(HPMCHistoPyramid* h)
 {
-    const uint32_t n = (uint32_t)h->n_level4;
     stage_mark(h, kStageScan);
-    const unsigned grid = h->n_tiles + h->n_ctiles;
-    hipLaunchKernelGGL(hpmc_scan_reduce, dim3(grid), dim3(256), 0, h->stream, h->geom, n, h->n_tiles,
-                       (const uint16_t*)h->colcnt, h->n_cc, h->tile_sums);
-    hipLaunchKernelGGL(hpmc_scan_scatter, dim3(grid), dim3(256), 0, h->stream, h->geom, n, h->n_tiles,
-                       (const uint16_t*)h->colcnt, h->n_cc, (const uint2*)h->tile_sums, h->n_ctiles, h->l4_off,
-                       h->items, h->d_totals, h->h_totals);
+    HIP_TRY(h, hpmc_host::plan_scan(h->plan, (const void*)hpmc_scan_reduce, (const void*)hpmc_scan_scatter,
+                                    h->stream));
     HIP_TRY(h, hipGetLastError());
     stage_mark(h, kStageScan);
     return true;
@@ -586,22 +451,10 @@ bool launch_scan(HPMCHistoPyramid* h)
 
 bool launch_emit(HPMCHistoPyramid* h, float* out, hpmc::CellEdge* out_ce, uint32_t capacity)
 {
-    const hpmc::Geom& g = h->geom;
-    const int grid = emit_grid(h);
-    hpmc::Geom ga = g;
-    float Ta = h->threshold;
-    const void* fa = field_source(h);
-    const uint2* li = h->items;
-    const uint32_t* lo = h->l4_off;
-    const hpmc::Totals* tt = h->d_totals;
-    float* o = out;
-    hpmc::CellEdge* oc = out_ce;
-    uint32_t cap = capacity;
-    const uint64_t* cs = h->constants->d_cases;
-    const float4* bn = h->constants->d_bnorm;
-    void* args[] = {&ga, &Ta, &fa, &li, &lo, &tt, &o, &oc, &cap, &cs, &bn};
     stage_mark(h, kStageEmit);
-    if (!launch_field_kernel(h, h->f_emit, h->k_emit, (unsigned)grid, 256, args)) return false;
+    HIP_TRY(h, hpmc_host::plan_emit(h->plan, h->k_emit, h->threshold, out, out_ce, capacity, h->constants->d_cases,
+                                    h->constants->d_bnorm, h->stream));
+    HIP_TRY(h, hipGetLastError());
     stage_mark(h, kStageEmit);
     return true;
 }
@@ -614,7 +467,7 @@ bool wait_count(HPMCHistoPyramid* h)
         return true;
     }
     HIP_TRY(h, hipEventSynchronize(h->ev_totals));
-    h->count = h->h_totals->vertices;
+    h->count = h->plan.h_totals->vertices;
     h->count_valid = true;
     return true;
 }
@@ -949,15 +802,20 @@ int32_t HPMCgetCellCodes(struct HPMCHistoPyramid* h, uint8_t* codes)
     if (!h || !codes) return fail(h, HPMC_ERR_NULL, "NULL argument");
     if (h->broken) return fail(h, HPMC_ERR_BROKEN, "h is broken");
     if (h->tainted && !setup(h)) return h->last_error;
-    const hpmc::Geom& g = h->geom;
+    if (h->mode == kFieldVolume) {
+        if (hpmc_host::plan_set_field(h->plan, 0, h->volume, h->stream) != hipSuccess)
+            return fail(h, HPMC_ERR_HIP, "descriptor update failed");
+    }
+    const hpmc::Geom& g = h->plan.vols[0].g;
     const uint64_t n = (uint64_t)g.cx * g.cy * g.cz;
     const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 65536);
-    hpmc::Geom ga = g;
+    const hpmc::VolDesc* vols = h->plan.d_vols;
+    uint32_t vol = 0;
     float Ta = h->binary_active ? 0.5f : h->threshold;
-    const void* fa = field_source(h);
     uint8_t* c = codes;
-    void* args[] = {&ga, &Ta, &fa, &c};
-    if (!launch_field_kernel(h, h->f_codes, h->k_codes, blocks, 256, args)) return h->last_error;
+    void* args[] = {&vols, &vol, &Ta, &c};
+    if (hpmc_host::launch(h->k_codes, blocks, 256, args, h->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+        return fail(h, HPMC_ERR_HIP, "codes kernel launch failed");
     ok(h);
     return HPMC_OK;
 }
@@ -972,18 +830,19 @@ uint64_t HPMCgetLevel4(struct HPMCHistoPyramid* h, uint32_t* out, uint64_t capac
         fail(h, HPMC_ERR_ARG, "HistoPyramid not built");
         return 0;
     }
-    if (capacity < h->n_level4) {
-        fail(h, HPMC_ERR_CAPACITY, "need %llu entries", (unsigned long long)h->n_level4);
+    const uint64_t n = h->plan.vols[0].n_level4;
+    if (capacity < n) {
+        fail(h, HPMC_ERR_CAPACITY, "need %llu entries", (unsigned long long)n);
         return 0;
     }
-    hipLaunchKernelGGL(hpmc_level4_gather, dim3(1024), dim3(256), 0, h->stream, h->geom,
-                       (const uint16_t*)h->colcnt, (uint32_t)h->n_level4, out);
+    hipLaunchKernelGGL(hpmc_level4_gather, dim3(1024), dim3(256), 0, h->stream, (const hpmc::VolDesc*)h->plan.d_vols,
+                       0u, (const uint16_t*)h->plan.colcnt, out);
     if (hipGetLastError() != hipSuccess) {
         fail(h, HPMC_ERR_HIP, "level-4 gather launch failed");
         return 0;
     }
     ok(h);
-    return h->n_level4;
+    return n;
 }
 
 int32_t HPMCsetProfiling(struct HPMCHistoPyramid* h, int32_t enable)

@@ -73,6 +73,35 @@ struct CellEdge {
     uint32_t edge;        // 4*axis + s0 + 2*s1
 };
 
+// One volume of a build.  A HistoPyramid is a batch of one volume; a batch
+// concatenates the per-volume arrays (column counts, Morton offsets, scan
+// tiles, classification workgroups) and records each volume's share here.
+struct VolDesc {
+    Geom g;
+    const void* field;        // FP32 (or RGBA) lattice; null for snippets
+    uint64_t colcnt_off;      // first u16 entry of the volume's column counts
+    uint32_t l4_base;         // first entry of its Morton-order block offsets
+    uint32_t n_level4;        // level-4 keys (side^2)
+    uint32_t n_cc;            // column-count entries
+    uint32_t cls_wg0;         // first logical classification workgroup
+    uint32_t l4_tile0;        // first level-4 scan tile (global index)
+    uint32_t n_l4tiles;
+    uint32_t col_tile0;       // first column scan tile (index among column tiles)
+    uint32_t n_coltiles;
+};
+
+// Vertex base and count of one volume of a batch.
+struct VolCount {
+    uint32_t base;
+    uint32_t count;
+};
+
+// Emit work item: x = level-4 key; y = chunk (bits 0..3, first vertex
+// chunk * kEmitChunk) | block vertex count (bits 4..15) | volume (16..31).
+__device__ __forceinline__ uint32_t item_chunk_first(uint32_t y) { return (y & 0xfu) << 8; }
+__device__ __forceinline__ uint32_t item_block_count(uint32_t y) { return (y >> 4) & 0xfffu; }
+__device__ __forceinline__ uint32_t item_volume(uint32_t y) { return y >> 16; }
+
 __device__ __forceinline__ uint32_t morton_spread(uint32_t v)
 {
     v &= 0xffffu;
@@ -312,22 +341,14 @@ __device__ __forceinline__ uint32_t classify_slice_counts(const ClassifyWave& w,
     return acc;
 }
 
-// Workgroup -> (x band, y band, z segment).  Workgroups are dealt to the
-// eight XCDs round robin; the mapping gives each XCD a contiguous run of
-// y bands, so workgroups resident together on one XCD are y-neighbours and
-// share their halo rows in that XCD's L2.
-__device__ __forceinline__ void classify_tile(const Geom& g, int zseg, int wgx, int& bx, int& py, int& pz)
+// Logical workgroup t of a volume -> (x band, y band, z segment), x fastest.
+// The launch deals workgroups to the eight XCDs round robin and maps them
+// to logical indices so that each XCD runs a contiguous run (classify
+// kernel wrapper): workgroups resident together on one XCD are y-neighbours
+// and share their halo rows in that XCD's L2.
+__device__ __forceinline__ void classify_tile(const Geom& g, int zseg, int wgx, int t, int& bx, int& py, int& pz)
 {
     const int npy = (g.cy + kBlock - 1) >> kBlockLog2;
-    const int npz = (g.cz + zseg - 1) / zseg;
-    const int total = wgx * npy * npz;
-    const int b = (int)blockIdx.x;
-    const int per_xcd = (total + 7) >> 3;
-    const int t = (b & 7) * per_xcd + (b >> 3);
-    if (t >= total) {
-        bx = -1;
-        return;
-    }
     bx = t % wgx;
     const int rest = t / wgx;
     py = rest % npy;
@@ -339,15 +360,14 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
                                               uint16_t* __restrict__ colcnt, int zseg,
                                               const uint8_t* __restrict__ lut,
                                               uint32_t (*xch)[kClassifyMaxWaves],
-                                              uint16_t (*stage)[4][kColStride])
+                                              uint16_t (*stage)[4][kColStride], int t)
 {
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nw = blockDim.x >> 6;
     const int wgx = (g.sx + 64 * nw - 1) / (64 * nw);    // workgroups across x
     int bx, py, pz;
-    classify_tile(g, zseg, wgx, bx, py, pz);
-    if (bx < 0) return;   // whole workgroup
+    classify_tile(g, zseg, wgx, t, bx, py, pz);
     bx = __builtin_amdgcn_readfirstlane(bx);
     py = __builtin_amdgcn_readfirstlane(py);
     pz = __builtin_amdgcn_readfirstlane(pz);
@@ -485,6 +505,7 @@ __device__ __forceinline__ void scan_load16(const uint32_t* __restrict__ level4,
 // Emit work items hold at most kEmitChunk vertices of one block, so a busy
 // block is spread over several waves.
 constexpr int kEmitChunk = 256;
+static_assert(kEmitChunk == 256, "item_chunk_first assumes 256-vertex chunks");
 
 __device__ __forceinline__ uint32_t chunks_of(uint32_t cnt) { return (cnt + kEmitChunk - 1) / kEmitChunk; }
 
@@ -520,81 +541,102 @@ __device__ __forceinline__ void scan_gather16(const Geom& g, const uint16_t* __r
     for (int k = 0; k < kScanPer; ++k) c[k] = (base + k < n) ? key_count(g, colcnt, base + k) : 0u;
 }
 
-// Tiles [0, ntiles) cover the level-4 keys (Morton order): sums of counts
-// (x) and of non-empty blocks (y).  Tiles [ntiles, ntiles + nctiles) cover
-// the column-major counts: vertices (x) and emit work items (y).
-__device__ __forceinline__ void scan_reduce_body(const Geom& g, uint32_t n, uint32_t ntiles,
-                                                 const uint16_t* __restrict__ colcnt, uint32_t ncc,
-                                                 uint2* __restrict__ tile_sums)
+// Scan tiles of a build: [0, n_l4t) are level-4 tiles (4096 Morton keys of
+// one volume each, volumes in order), [n_l4t, n_l4t + n_colt) column tiles
+// (4096 column-count entries of one volume each).  tile_vol[tile] names the
+// volume.  Level-4 tiles sum vertices (x) and non-empty blocks (y); column
+// tiles sum vertices (x) and emit work items (y).
+__device__ __forceinline__ void scan_load_tile(const VolDesc& d, const uint16_t* __restrict__ colcnt, bool is_col,
+                                               uint32_t local_tile, uint32_t (&c)[kScanPer])
 {
-    __shared__ uint2 sh[16];
-    uint2 s = make_uint2(0, 0);
-    uint32_t c[kScanPer];
-    if (blockIdx.x < ntiles) {
-        scan_gather16(g, colcnt, n, blockIdx.x * kScanTile + threadIdx.x * kScanPer, c);
-#pragma unroll
-        for (int k = 0; k < kScanPer; ++k) {
-            s.x += c[k];
-            s.y += c[k] != 0;
-        }
-    } else {
-        scan_load16_u16(colcnt, ncc, (blockIdx.x - ntiles) * kScanTile + threadIdx.x * kScanPer, c);
-#pragma unroll
-        for (int k = 0; k < kScanPer; ++k) {
-            s.x += c[k];
-            s.y += chunks_of(c[k]);
-        }
-    }
-    s = block_reduce_u2(s, sh);
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = s;
+    const uint32_t base = local_tile * kScanTile + threadIdx.x * kScanPer;
+    if (!is_col) scan_gather16(d.g, colcnt + d.colcnt_off, d.n_level4, base, c);
+    else scan_load16_u16(colcnt + d.colcnt_off, d.n_cc, base, c);
 }
 
-// Level-4 tiles: every tile re-reads the preceding tile sums (a few
-// hundred), scans its own 4096 entries and writes, for each non-empty
-// block, the exclusive vertex offset of its first vertex in Morton order
-// (the reference's key order).  Tile 0 also writes the totals, to device
-// memory and to the host-visible copy.
-// Column tiles: the same scan over the column-major counts, writing the
-// emit work items -- (block key, first vertex | count << 16), one per
-// kEmitChunk vertices of a block -- in (z segment, y, x, z) order, so that
-// consecutive items are consecutive slices of one block column.
-__device__ __forceinline__ void scan_scatter_body(const Geom& g, uint32_t n,
-                                                  uint32_t ntiles, const uint16_t* __restrict__ colcnt, uint32_t ncc,
-                                                  const uint2* __restrict__ tile_sums, uint32_t nctiles,
+__device__ __forceinline__ void scan_reduce_body(const VolDesc* __restrict__ vols,
+                                                 const uint32_t* __restrict__ tile_vol, uint32_t n_l4t,
+                                                 const uint16_t* __restrict__ colcnt, uint2* __restrict__ tile_sums)
+{
+    __shared__ uint2 sh[16];
+    const uint32_t tile = blockIdx.x;
+    const bool is_col = tile >= n_l4t;
+    const VolDesc d = vols[__builtin_amdgcn_readfirstlane(tile_vol[tile])];
+    const uint32_t local = is_col ? tile - n_l4t - d.col_tile0 : tile - d.l4_tile0;
+    uint32_t c[kScanPer];
+    scan_load_tile(d, colcnt, is_col, local, c);
+    uint2 s = make_uint2(0, 0);
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        s.x += c[k];
+        s.y += is_col ? chunks_of(c[k]) : (uint32_t)(c[k] != 0);
+    }
+    s = block_reduce_u2(s, sh);
+    if (threadIdx.x == 0) tile_sums[tile] = s;
+}
+
+// Every tile re-reads the sums of the tiles before it in its sequence (a
+// few hundred) and scans its own 4096 entries.  Level-4 tiles write, for
+// each non-empty block, the global offset of its first vertex: the
+// concatenation of the volumes' vertex lists, each in Morton key order (the
+// reference's key order).  The first level-4 tile of a volume also writes
+// the volume's vertex base and count; tile 0 the totals, to device memory
+// and to the host-visible copies.  Column tiles write the emit work items --
+// one per kEmitChunk vertices of a block -- in (volume, z segment, y, x, z)
+// order, so that consecutive items are consecutive slices of one block column.
+__device__ __forceinline__ void scan_scatter_body(const VolDesc* __restrict__ vols,
+                                                  const uint32_t* __restrict__ tile_vol, uint32_t n_l4t,
+                                                  uint32_t n_colt, const uint16_t* __restrict__ colcnt,
+                                                  const uint2* __restrict__ tile_sums,
                                                   uint32_t* __restrict__ l4_off, uint2* __restrict__ items,
-                                                  Totals* __restrict__ totals, Totals* __restrict__ host_totals)
+                                                  Totals* __restrict__ totals, Totals* __restrict__ host_totals,
+                                                  VolCount* __restrict__ vol_counts,
+                                                  VolCount* __restrict__ host_vol_counts)
 {
     __shared__ uint2 sh[16];
     __shared__ uint2 wsum[16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const bool is_col = blockIdx.x >= ntiles;
-    const uint32_t t0 = is_col ? ntiles : 0u;
-    const uint32_t tile = blockIdx.x - t0;
-    const uint32_t base = tile * kScanTile + threadIdx.x * kScanPer;
+    const uint32_t tile = blockIdx.x;
+    const bool is_col = tile >= n_l4t;
+    const uint32_t vol = __builtin_amdgcn_readfirstlane(tile_vol[tile]);
+    const VolDesc d = vols[vol];
+    const uint32_t t0 = is_col ? n_l4t : 0u;
+    const uint32_t local = is_col ? tile - n_l4t - d.col_tile0 : tile - d.l4_tile0;
     uint32_t c[kScanPer];
+    scan_load_tile(d, colcnt, is_col, local, c);
     uint2 mine = make_uint2(0, 0);
-    if (!is_col) {
-        scan_gather16(g, colcnt, n, base, c);
 #pragma unroll
-        for (int k = 0; k < kScanPer; ++k) mine.x += c[k];
-    } else {
-        scan_load16_u16(colcnt, ncc, base, c);
-#pragma unroll
-        for (int k = 0; k < kScanPer; ++k) mine.y += chunks_of(c[k]);
+    for (int k = 0; k < kScanPer; ++k) {
+        if (!is_col) mine.x += c[k];
+        else mine.y += chunks_of(c[k]);
     }
+    // sums of the preceding tiles of this sequence
     uint2 pre = make_uint2(0, 0);
-    for (uint32_t i = threadIdx.x; i < tile; i += blockDim.x) {
-        pre.x += tile_sums[t0 + i].x;
-        pre.y += tile_sums[t0 + i].y;
+    for (uint32_t i = t0 + threadIdx.x; i < tile; i += blockDim.x) {
+        pre.x += tile_sums[i].x;
+        pre.y += tile_sums[i].y;
     }
     pre = block_reduce_u2(pre, sh);
-    if (blockIdx.x == 0) {
+    if (!is_col && local == 0) {
+        // this volume's vertex base and count
+        uint32_t cnt = 0;
+        for (uint32_t i = threadIdx.x; i < d.n_l4tiles; i += blockDim.x) cnt += tile_sums[d.l4_tile0 + i].x;
+        cnt = block_reduce_u2(make_uint2(cnt, 0), sh).x;
+        if (threadIdx.x == 0) {
+            VolCount vc;
+            vc.base = pre.x;
+            vc.count = cnt;
+            vol_counts[vol] = vc;
+            if (host_vol_counts) host_vol_counts[vol] = vc;
+        }
+    }
+    if (tile == 0) {
         uint2 tot = make_uint2(0, 0), items_tot = make_uint2(0, 0);
-        for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) {
+        for (uint32_t i = threadIdx.x; i < n_l4t; i += blockDim.x) {
             tot.x += tile_sums[i].x;
             tot.y += tile_sums[i].y;
         }
-        for (uint32_t i = threadIdx.x; i < nctiles; i += blockDim.x) items_tot.y += tile_sums[ntiles + i].y;
+        for (uint32_t i = threadIdx.x; i < n_colt; i += blockDim.x) items_tot.y += tile_sums[n_l4t + i].y;
         tot = block_reduce_u2(tot, sh);
         items_tot = block_reduce_u2(items_tot, sh);
         if (threadIdx.x == 0) {
@@ -623,13 +665,15 @@ __device__ __forceinline__ void scan_scatter_body(const Geom& g, uint32_t n,
         off.x += wsum[i].x;
         off.y += wsum[i].y;
     }
+    const uint32_t base = local * kScanTile + threadIdx.x * kScanPer;
     if (!is_col) {
 #pragma unroll
         for (int k = 0; k < kScanPer; ++k) {
-            if (c[k]) l4_off[base + k] = off.x;
+            if (c[k]) l4_off[d.l4_base + base + k] = off.x;
             off.x += c[k];
         }
     } else {
+        const Geom& g = d.g;
 #pragma unroll
         for (int k = 0; k < kScanPer; ++k) {
             if (!c[k]) continue;
@@ -641,7 +685,8 @@ __device__ __forceinline__ void scan_scatter_body(const Geom& g, uint32_t n,
             const int by = (int)(rest % (uint32_t)g.nby);
             const int pz = (int)(rest / (uint32_t)g.nby);
             const uint32_t key = block_key(g, bxb << kBlockLog2, by << kBlockLog2, pz * g.zseg + zi);
-            for (uint32_t v = 0; v < c[k]; v += kEmitChunk) items[off.y++] = make_uint2(key, v | (c[k] << 16));
+            const uint32_t yv = (c[k] << 4) | (vol << 16);
+            for (uint32_t ch = 0; ch * kEmitChunk < c[k]; ++ch) items[off.y++] = make_uint2(key, yv | ch);
         }
     }
 }
@@ -755,8 +800,11 @@ __device__ __forceinline__ void emit_store(EmitWaveShared& ws, int p_lo, int p_h
     }
 }
 
-template <int MODE, class F>
-__device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T,
+// BATCH: items of several volumes (the volume of each item is looked up);
+// otherwise every item belongs to vols[0].  mk(d) makes the field of
+// volume descriptor d.
+template <int MODE, bool BATCH, class MK>
+__device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, const MK& mk, float T,
                                           const uint2* __restrict__ items, const uint32_t* __restrict__ l4_off,
                                           const Totals* __restrict__ totals,
                                           float* __restrict__ out, CellEdge* __restrict__ out_ce,
@@ -784,17 +832,20 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
     uint2 cur = items[it];
     cur.x = __builtin_amdgcn_readfirstlane(cur.x);
     cur.y = __builtin_amdgcn_readfirstlane(cur.y);
-    uint32_t obase = l4_off[cur.x];
+    VolDesc dn = vols[BATCH ? item_volume(cur.y) : 0u];
+    uint32_t obase = l4_off[dn.l4_base + cur.x];
     float r[3][kPlaneLoads];
     int x0, y0, z;
-    block_origin(g, cur.x, x0, y0, z);
-    emit_fetch(field, x0, y0, z, z + NP - 1, lane, r);
+    block_origin(dn.g, cur.x, x0, y0, z);
+    emit_fetch(mk(dn), x0, y0, z, z + NP - 1, lane, r);
     uint2 nxt = items[min(it + stride, nitems - 1)];
 
     for (;;) {
-        const uint32_t vlo = cur.y & 0xffffu, cnt = cur.y >> 16;
+        const uint32_t vlo = item_chunk_first(cur.y), cnt = item_block_count(cur.y);
         const uint32_t base = __builtin_amdgcn_readfirstlane(obase);
         const int cz = z, cx0 = x0, cy0 = y0;
+        const VolDesc dc = dn;          // this item's volume
+        const Geom& g = dc.g;
         wave_lds_sync();
         emit_store(ws, 0, NP - 1, lane, r);
         wave_lds_sync();
@@ -805,9 +856,10 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
         if (more) {
             nxt.x = __builtin_amdgcn_readfirstlane(nxt.x);
             nxt.y = __builtin_amdgcn_readfirstlane(nxt.y);
-            obase = l4_off[nxt.x];
-            block_origin(g, nxt.x, x0, y0, z);
-            emit_fetch(field, x0, y0, z, z + NP - 1, lane, r);
+            if constexpr (BATCH) dn = vols[item_volume(nxt.y)];
+            obase = l4_off[dn.l4_base + nxt.x];
+            block_origin(dn.g, nxt.x, x0, y0, z);
+            emit_fetch(mk(dn), x0, y0, z, z + NP - 1, lane, r);
             after = items[min(it + 2 * stride, nitems - 1)];
         }
 
@@ -890,6 +942,7 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
                 nz = bn.z;
             } else if constexpr (MODE == kEmitGradient) {
                 // shadergen.cpp:539-549: values and gradients at both ends
+                const auto field = mk(dc);
                 const float4 fa = field.grad(cx0 + ax, cy0 + ay, cz + az);
                 const float4 fb = field.grad(cx0 + bx_, cy0 + by_, cz + bz_);
                 tt = (fa.w - T) * __builtin_amdgcn_rcpf(fa.w - fb.w);
@@ -936,11 +989,11 @@ __device__ __forceinline__ void emit_body(const F& field, const Geom& g, float T
 }
 
 // The level-4 counts in Morton key order (parity and debugging).
-__device__ __forceinline__ void level4_body(const Geom& g, const uint16_t* __restrict__ colcnt, uint32_t n,
+__device__ __forceinline__ void level4_body(const VolDesc& d, const uint16_t* __restrict__ colcnt,
                                             uint32_t* __restrict__ out)
 {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        out[k] = key_count(g, colcnt, k);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < d.n_level4; k += gridDim.x * blockDim.x)
+        out[k] = key_count(d.g, colcnt + d.colcnt_off, k);
 }
 
 // Per-cell codes of the whole grid (parity and debugging).
@@ -963,30 +1016,37 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
 }  // namespace hpmc
 
 // Kernel entry points for one field source.  MAKE_FIELD builds the field
-// object from the Geom `g` and the opaque source pointer `fsrc`.
+// object from the volume descriptor `dd` (geometry and field pointer).
 // HPMC_DEFINE_BUILD_KERNELS: classification and the per-cell code readback.
 // HPMC_DEFINE_EMIT_KERNEL: one extraction kernel in emission mode MODE.
 #define HPMC_DEFINE_BUILD_KERNELS(SUFFIX, MAKE_FIELD)                                          \
     extern "C" __global__ __launch_bounds__(1024) void hpmc_classify_##SUFFIX(                 \
-        hpmc::Geom g, float T, const void* fsrc, uint16_t* colcnt, int zseg,                  \
-        const uint64_t* cases)                                                                \
+        const hpmc::VolDesc* vols, const uint32_t* cls_vol, uint32_t n_wg, float T,            \
+        uint16_t* colcnt, const uint64_t* cases)                                              \
     {                                                                                         \
         __shared__ uint8_t lut[256];                                                          \
         __shared__ uint32_t xch[2][hpmc::kClassifyMaxWaves];                                   \
         __shared__ uint16_t stage[hpmc::kClassifyMaxWaves][4][hpmc::kColStride];               \
         for (int i = threadIdx.x; i < 256; i += blockDim.x) lut[i] = (uint8_t)(cases[i] & 0xfu); \
         __syncthreads();                                                                      \
-        hpmc::classify_body(MAKE_FIELD, g, T, colcnt, zseg, lut, xch, stage);                 \
+        /* grid padded to a multiple of 8: XCD x takes logical run x */                       \
+        const uint32_t t = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);          \
+        if (t >= n_wg) return;                                                                \
+        const hpmc::VolDesc dd = vols[__builtin_amdgcn_readfirstlane(cls_vol[t])];            \
+        hpmc::classify_body(MAKE_FIELD, dd.g, T, colcnt + dd.colcnt_off, dd.g.zseg, lut, xch, \
+                            stage, (int)(t - dd.cls_wg0));                                    \
     }                                                                                         \
     extern "C" __global__ __launch_bounds__(256) void hpmc_codes_##SUFFIX(                     \
-        hpmc::Geom g, float T, const void* fsrc, uint8_t* codes)                              \
+        const hpmc::VolDesc* vols, uint32_t vol, float T, uint8_t* codes)                     \
     {                                                                                         \
-        hpmc::codes_body(MAKE_FIELD, g, T, codes);                                            \
+        const hpmc::VolDesc dd = vols[vol];                                                   \
+        hpmc::codes_body(MAKE_FIELD, dd.g, T, codes);                                         \
     }
 
-#define HPMC_DEFINE_EMIT_KERNEL(NAME, MAKE_FIELD, MODE)                                        \
+// BATCH: items of many volumes (the batched build) or of vols[0] only.
+#define HPMC_DEFINE_EMIT_KERNEL(NAME, MAKE_FIELD, MODE, BATCH)                                 \
     extern "C" __global__ __launch_bounds__(256) void NAME(                                   \
-        hpmc::Geom g, float T, const void* fsrc, const uint2* items, const uint32_t* l4_off,   \
+        const hpmc::VolDesc* vols, float T, const uint2* items, const uint32_t* l4_off,        \
         const hpmc::Totals* totals, float* out, hpmc::CellEdge* out_ce, uint32_t capacity,    \
         const uint64_t* cases, const float4* bnorm)                                           \
     {                                                                                         \
@@ -994,7 +1054,7 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
         sh.cases[threadIdx.x] = cases[threadIdx.x];                                           \
         sh.lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);                           \
         __syncthreads();                                                                      \
-        hpmc::emit_body<MODE>(MAKE_FIELD, g, T, items, l4_off, totals, out, out_ce, capacity, \
-                              bnorm, sh);                                                     \
+        auto mk = [](const hpmc::VolDesc& dd) { return MAKE_FIELD; };                         \
+        hpmc::emit_body<MODE, BATCH>(vols, mk, T, items, l4_off, totals, out, out_ce,         \
+                                     capacity, bnorm, sh);                                    \
     }
-
