@@ -69,6 +69,19 @@ SIGNATURES = [
     ("HPMCgetStageTimes", _i32, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u32)]),
     ("HPMCcheckFieldSnippet", _i32, [ctypes.c_char_p, _u32, ctypes.c_char_p, ctypes.c_size_t]),
     ("HPMCgetLayout", _i32, [_vp, ctypes.POINTER(_i32)]),
+    ("HPMCcreateBatch", _vp, [_vp]),
+    ("HPMCdestroyBatch", None, [_vp]),
+    ("HPMCbatchAddVolume", _i32, [_vp, _vp, _i32, _i32, _i32, _f, _f, _f]),
+    ("HPMCbatchNumVolumes", _i32, [_vp]),
+    ("HPMCbatchSetTuning", _i32, [_vp, _i32, _i32, _i32]),
+    ("HPMCbatchGetTuning", _i32, [_vp, ctypes.POINTER(_i32)]),
+    ("HPMCbatchSetStream", _i32, [_vp, _vp]),
+    ("HPMCbatchBuild", _i32, [_vp, _f]),
+    ("HPMCbatchGetCounts", _i32, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u64)]),
+    ("HPMCbatchExtract", _i32, [_vp, _vp, _u64, _vp]),
+    ("HPMCbatchCaptureGraph", _i32, [_vp, _f, _vp, _u64, _vp]),
+    ("HPMCbatchLaunchGraph", _i32, [_vp]),
+    ("HPMCbatchAutotune", _i32, [_vp, _f, _vp, _u64, _i32, ctypes.POINTER(ctypes.c_double)]),
 ]
 
 _LIB = None
@@ -271,6 +284,98 @@ class IsoSurface:
         if self.h:
             L.HPMCdestroyHandle(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many FP32 volumes built and extracted together (HPMCBatch*): one
+    classification, one scan and one extraction launch over all volumes,
+    optionally replayed as a hipGraph.  Per-volume results equal those of
+    IsoSurface on each volume alone; the output concatenates them in the
+    order the volumes were added."""
+
+    _constants = None
+
+    def __init__(self, stream=None):
+        L = lib()
+        if IsoSurface._constants is None:
+            c = L.HPMCcreateConstants(4, 3)
+            if not c:
+                raise HPMCError("HPMCcreateConstants failed: %s" % (last_error()[1],))
+            IsoSurface._constants = c
+        self.b = L.HPMCcreateBatch(IsoSurface._constants)
+        if not self.b:
+            raise HPMCError("HPMCcreateBatch failed: %s" % (last_error()[1],))
+        self._volumes = []
+        if stream is not None:
+            _check(L.HPMCbatchSetStream(self.b, _stream_ptr(stream)))
+
+    def add_volume(self, volume, extent=(1.0, 1.0, 1.0)):
+        """volume: contiguous float32 device tensor (z, y, x); returns its index."""
+        import torch
+        assert volume.dtype == torch.float32 and volume.is_contiguous() and volume.is_cuda
+        z, y, x = volume.shape
+        i = lib().HPMCbatchAddVolume(self.b, ctypes.c_void_p(volume.data_ptr()), x, y, z, *[float(e) for e in extent])
+        if i < 0:
+            _check(-i)
+        self._volumes.append(volume)
+        return i
+
+    def __len__(self):
+        return len(self._volumes)
+
+    def set_tuning(self, classify_waves=0, zseg=0, emit_workgroups=0):
+        _check(lib().HPMCbatchSetTuning(self.b, classify_waves, zseg, emit_workgroups))
+
+    def tuning(self):
+        out = (_i32 * 3)()
+        _check(lib().HPMCbatchGetTuning(self.b, out))
+        return tuple(out)
+
+    def build(self, iso):
+        _check(lib().HPMCbatchBuild(self.b, float(iso)))
+
+    def counts(self):
+        """(counts[n], bases[n], total) of the last build (waits)."""
+        import numpy as np
+        n = len(self._volumes)
+        counts = np.zeros(n, np.uint32)
+        bases = np.zeros(n, np.uint32)
+        total = _u64(0)
+        _check(lib().HPMCbatchGetCounts(self.b, counts.ctypes.data_as(ctypes.POINTER(_u32)),
+                                        bases.ctypes.data_as(ctypes.POINTER(_u32)), ctypes.byref(total)))
+        return counts, bases, int(total.value)
+
+    def extract(self, out, cell_edge=None):
+        """out: float32 (capacity, 6) device tensor; cell_edge: int32 (capacity, 2) or None."""
+        ce = ctypes.c_void_p(cell_edge.data_ptr()) if cell_edge is not None else None
+        cap = out.shape[0] if cell_edge is None else min(out.shape[0], cell_edge.shape[0])
+        _check(lib().HPMCbatchExtract(self.b, ctypes.c_void_p(out.data_ptr()), cap, ce))
+
+    def capture_graph(self, iso, out, cell_edge=None):
+        ce = ctypes.c_void_p(cell_edge.data_ptr()) if cell_edge is not None else None
+        cap = out.shape[0] if cell_edge is None else min(out.shape[0], cell_edge.shape[0])
+        _check(lib().HPMCbatchCaptureGraph(self.b, float(iso), ctypes.c_void_p(out.data_ptr()), cap, ce))
+
+    def launch_graph(self):
+        _check(lib().HPMCbatchLaunchGraph(self.b))
+
+    def autotune(self, iso, out, repetitions=3):
+        """Picks the fastest launch shapes for build + extract; returns (shape, ms)."""
+        best = ctypes.c_double(0.0)
+        _check(lib().HPMCbatchAutotune(self.b, float(iso), ctypes.c_void_p(out.data_ptr()), out.shape[0],
+                                       repetitions, ctypes.byref(best)))
+        return self.tuning(), best.value
+
+    def close(self):
+        if self.b:
+            lib().HPMCdestroyBatch(self.b)
+            self.b = None
 
     def __del__(self):
         try:

@@ -907,3 +907,313 @@ int32_t HPMCgetLayout(struct HPMCHistoPyramid* h, int32_t out[5])
 }
 
 }  // extern "C"
+
+// =========================================================================
+// Batched builds: one plan over many volumes.
+struct HPMCBatch {
+    HPMCConstants* constants = nullptr;
+    std::vector<hpmc_host::VolumeSpec> specs;
+    hpmc_host::Tuning tuning;
+    bool tainted = true;
+    hpmc_host::Plan plan;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool built = false;
+    bool counts_known = false;
+    float last_threshold = 0.0f;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+namespace {
+
+#define BATCH_TRY(expr)                                                                    \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess) return fail(nullptr, HPMC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+void batch_drop_graph(HPMCBatch* b)
+{
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+}
+
+int32_t batch_setup(HPMCBatch* b)
+{
+    if (!b->stream) {
+        // blocking with respect to the null stream, default priority
+        BATCH_TRY(hipStreamCreate(&b->stream));
+        b->own_stream = true;
+    }
+    if (!b->tainted) return HPMC_OK;
+    batch_drop_graph(b);
+    std::string err;
+    const int32_t rc = hpmc_host::plan_create(b->plan, b->specs, b->tuning, (const void*)hpmc_classify_volume,
+                                              device_shape(), &err);
+    if (rc != HPMC_OK) return fail(nullptr, rc, "%s", err.c_str());
+    b->tainted = false;
+    b->built = false;
+    b->counts_known = false;
+    return HPMC_OK;
+}
+
+int32_t batch_enqueue_build(HPMCBatch* b, float T)
+{
+    BATCH_TRY(hpmc_host::plan_classify(b->plan, hpmc_host::Kernel{(const void*)hpmc_classify_volume, nullptr}, T,
+                                       b->constants->d_cases, b->stream));
+    BATCH_TRY(hpmc_host::plan_scan(b->plan, (const void*)hpmc_scan_reduce, (const void*)hpmc_scan_scatter,
+                                   b->stream));
+    return HPMC_OK;
+}
+
+int32_t batch_enqueue_extract(HPMCBatch* b, float T, float* out, uint64_t capacity, void* cell_edge)
+{
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(capacity, 0xffffffffull);
+    BATCH_TRY(hpmc_host::plan_emit(b->plan, hpmc_host::Kernel{(const void*)hpmc_emit_volume_batch, nullptr}, T, out,
+                                   (hpmc::CellEdge*)cell_edge, cap, b->constants->d_cases, b->constants->d_bnorm,
+                                   b->stream));
+    return HPMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+struct HPMCBatch* HPMCcreateBatch(struct HPMCConstants* c)
+{
+    if (!c) {
+        fail(nullptr, HPMC_ERR_NULL, "createBatch called with constants = NULL");
+        return nullptr;
+    }
+    HPMCBatch* b = new (std::nothrow) HPMCBatch;
+    if (!b) return nullptr;
+    b->constants = c;
+    ok(nullptr);
+    return b;
+}
+
+void HPMCdestroyBatch(struct HPMCBatch* b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    hpmc_host::plan_free(b->plan);
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int32_t HPMCbatchAddVolume(struct HPMCBatch* b, const float* volume, int32_t x, int32_t y, int32_t z, float ext_x,
+                           float ext_y, float ext_z)
+{
+    if (!b || !volume) return -fail(nullptr, HPMC_ERR_NULL, "NULL argument");
+    hpmc_host::VolumeSpec v;
+    v.size[0] = x;
+    v.size[1] = y;
+    v.size[2] = z;
+    for (int a = 0; a < 3; ++a) v.cells[a] = v.size[a] - 1;
+    v.extent[0] = ext_x;
+    v.extent[1] = ext_y;
+    v.extent[2] = ext_z;
+    v.field = volume;
+    hpmc_host::Layout lay;
+    std::string err;
+    const int32_t rc = hpmc_host::layout_of(v.size, v.cells, &lay, &err);
+    if (rc != HPMC_OK) return -fail(nullptr, rc, "%s", err.c_str());
+    if (b->specs.size() >= 65535) return -fail(nullptr, HPMC_ERR_CAPACITY, "a batch holds at most 65535 volumes");
+    b->specs.push_back(v);
+    b->tainted = true;
+    ok(nullptr);
+    return (int32_t)b->specs.size() - 1;
+}
+
+int32_t HPMCbatchNumVolumes(struct HPMCBatch* b) { return b ? (int32_t)b->specs.size() : 0; }
+
+int32_t HPMCbatchSetTuning(struct HPMCBatch* b, int32_t classify_waves, int32_t zseg, int32_t emit_workgroups)
+{
+    if (!b) return fail(nullptr, HPMC_ERR_NULL, "NULL batch");
+    if (classify_waves < 0 || classify_waves > hpmc::kClassifyMaxWaves || (zseg && zseg != 8 && zseg != 16 &&
+                                                                           zseg != 32) || emit_workgroups < 0)
+        return fail(nullptr, HPMC_ERR_ARG, "invalid tuning (waves 0..16, zseg 0/8/16/32, workgroups >= 0)");
+    b->tuning.classify_waves = classify_waves;
+    b->tuning.zseg = zseg;
+    b->tuning.emit_wgs = emit_workgroups;
+    b->tainted = true;
+    ok(nullptr);
+    return HPMC_OK;
+}
+
+int32_t HPMCbatchGetTuning(struct HPMCBatch* b, int32_t out[3])
+{
+    if (!b || !out) return fail(nullptr, HPMC_ERR_NULL, "NULL argument");
+    if (b->specs.empty()) return fail(nullptr, HPMC_ERR_ARG, "empty batch");
+    const int32_t rc = batch_setup(b);
+    if (rc != HPMC_OK) return rc;
+    out[0] = b->plan.cls_waves;
+    out[1] = b->plan.vols[0].g.zseg;
+    out[2] = b->plan.emit_wgs;
+    ok(nullptr);
+    return HPMC_OK;
+}
+
+int32_t HPMCbatchSetStream(struct HPMCBatch* b, void* stream)
+{
+    if (!b) return fail(nullptr, HPMC_ERR_NULL, "NULL batch");
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
+    b->own_stream = false;
+    b->stream = (hipStream_t)stream;
+    batch_drop_graph(b);
+    ok(nullptr);
+    return HPMC_OK;
+}
+
+int32_t HPMCbatchBuild(struct HPMCBatch* b, float threshold)
+{
+    if (!b) return fail(nullptr, HPMC_ERR_NULL, "NULL batch");
+    if (b->specs.empty()) return fail(nullptr, HPMC_ERR_ARG, "empty batch");
+    int32_t rc = batch_setup(b);
+    if (rc != HPMC_OK) return rc;
+    rc = batch_enqueue_build(b, threshold);
+    if (rc != HPMC_OK) return rc;
+    b->last_threshold = threshold;
+    b->built = true;
+    b->counts_known = false;
+    ok(nullptr);
+    return HPMC_OK;
+}
+
+int32_t HPMCbatchGetCounts(struct HPMCBatch* b, uint32_t* counts, uint32_t* bases, uint64_t* total)
+{
+    if (!b) return fail(nullptr, HPMC_ERR_NULL, "NULL batch");
+    if (!b->built) return fail(nullptr, HPMC_ERR_ARG, "batch not built");
+    BATCH_TRY(hipStreamSynchronize(b->stream));
+    uint64_t sum = 0;
+    for (int v = 0; v < b->plan.nvol; ++v) {
+        if (counts) counts[v] = b->plan.h_vcounts[v].count;
+        if (bases) bases[v] = b->plan.h_vcounts[v].base;
+        sum += b->plan.h_vcounts[v].count;
+    }
+    if (total) *total = sum;
+    b->counts_known = true;
+    ok(nullptr);
+    return HPMC_OK;
+}
+
+int32_t HPMCbatchExtract(struct HPMCBatch* b, float* out, uint64_t capacity, void* cell_edge)
+{
+    if (!b || !out) return fail(nullptr, HPMC_ERR_NULL, "NULL argument");
+    if (!b->built || b->tainted) return fail(nullptr, HPMC_ERR_ARG, "batch not built since it was configured");
+    if (b->counts_known && b->plan.h_totals->vertices > capacity)
+        return fail(nullptr, HPMC_ERR_CAPACITY, "output holds %llu vertices, the batch has %u",
+                    (unsigned long long)capacity, b->plan.h_totals->vertices);
+    const int32_t rc = batch_enqueue_extract(b, b->last_threshold, out, capacity, cell_edge);
+    if (rc != HPMC_OK) return rc;
+    ok(nullptr);
+    return HPMC_OK;
+}
+
+int32_t HPMCbatchCaptureGraph(struct HPMCBatch* b, float threshold, float* out, uint64_t capacity, void* cell_edge)
+{
+    if (!b || !out) return fail(nullptr, HPMC_ERR_NULL, "NULL argument");
+    if (b->specs.empty()) return fail(nullptr, HPMC_ERR_ARG, "empty batch");
+    int32_t rc = batch_setup(b);
+    if (rc != HPMC_OK) return rc;
+    if (!b->stream) return fail(nullptr, HPMC_ERR_ARG, "graph capture needs a stream");
+    batch_drop_graph(b);
+    BATCH_TRY(hipStreamSynchronize(b->stream));
+    BATCH_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    b->last_threshold = threshold;
+    rc = batch_enqueue_build(b, threshold);
+    if (rc == HPMC_OK) rc = batch_enqueue_extract(b, threshold, out, capacity, cell_edge);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc != HPMC_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(nullptr, HPMC_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = g;
+    BATCH_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    ok(nullptr);
+    return HPMC_OK;
+}
+
+int32_t HPMCbatchLaunchGraph(struct HPMCBatch* b)
+{
+    if (!b) return fail(nullptr, HPMC_ERR_NULL, "NULL batch");
+    if (!b->graph_exec || b->tainted) return fail(nullptr, HPMC_ERR_ARG, "no captured graph for this configuration");
+    BATCH_TRY(hipGraphLaunch(b->graph_exec, b->stream));
+    b->built = true;
+    b->counts_known = false;
+    ok(nullptr);
+    return HPMC_OK;
+}
+
+int32_t HPMCbatchAutotune(struct HPMCBatch* b, float threshold, float* out, uint64_t capacity, int32_t repetitions,
+                          double* best_ms)
+{
+    if (!b || !out) return fail(nullptr, HPMC_ERR_NULL, "NULL argument");
+    if (b->specs.empty()) return fail(nullptr, HPMC_ERR_ARG, "empty batch");
+    if (repetitions < 1) repetitions = 1;
+    int max_row_waves = 1;
+    for (const auto& v : b->specs) max_row_waves = std::max(max_row_waves, hpmc_host::row_waves(v.size[0]));
+    std::vector<int> waves = {1};
+    for (int w = 2; w <= max_row_waves; w *= 2) waves.push_back(w);
+    if (std::find(waves.begin(), waves.end(), max_row_waves) == waves.end()) waves.push_back(max_row_waves);
+    const hpmc_host::DeviceShape& dev = device_shape();
+    const int full = (dev.cus > 0 ? dev.cus : 256) * (dev.emit_per_cu > 0 ? dev.emit_per_cu : 4);
+    const int emit_opts[] = {full / 2, full};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BATCH_TRY(hipEventCreate(&e0));
+    BATCH_TRY(hipEventCreate(&e1));
+    hpmc_host::Tuning best;
+    double best_t = 1e30;
+    int32_t rc = HPMC_OK;
+    for (int w : waves) {
+        for (int zs : {8, 16, 32}) {
+            for (int ew : emit_opts) {
+                b->tuning.classify_waves = w;
+                b->tuning.zseg = zs;
+                b->tuning.emit_wgs = ew;
+                b->tainted = true;
+                rc = batch_setup(b);
+                if (rc != HPMC_OK) break;
+                std::vector<float> ts;
+                for (int r = 0; r <= repetitions && rc == HPMC_OK; ++r) {
+                    (void)hipEventRecord(e0, b->stream);
+                    rc = batch_enqueue_build(b, threshold);
+                    if (rc == HPMC_OK) rc = batch_enqueue_extract(b, threshold, out, capacity, nullptr);
+                    (void)hipEventRecord(e1, b->stream);
+                    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(nullptr, HPMC_ERR_HIP, "autotune run failed");
+                    float ms = 0.0f;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    if (r > 0) ts.push_back(ms);   // the first run warms up
+                }
+                if (rc != HPMC_OK) break;
+                std::sort(ts.begin(), ts.end());
+                const double med = ts[ts.size() / 2];
+                if (med < best_t) {
+                    best_t = med;
+                    best = b->tuning;
+                }
+            }
+            if (rc != HPMC_OK) break;
+        }
+        if (rc != HPMC_OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != HPMC_OK) return rc;
+    b->tuning = best;
+    b->tainted = true;
+    rc = batch_setup(b);
+    if (rc != HPMC_OK) return rc;
+    if (best_ms) *best_ms = best_t;
+    ok(nullptr);
+    return HPMC_OK;
+}
+
+}  // extern "C"

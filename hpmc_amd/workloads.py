@@ -164,6 +164,32 @@ def metaballs(size, balls, extra=0):
     return s.astype(np.float32)
 
 
+def metaballs_torch(size, balls, device, extra=0):
+    """metaballs() evaluated with torch on `device`: the same FP32
+    elementwise operations in the same order (bit-identical)."""
+    import torch
+    X, Y, Z = (v + extra for v in size)
+    f32 = torch.float32
+
+    def centres(n, count):
+        inv = torch.tensor(1.0, dtype=f32) / torch.tensor(float(n), dtype=f32)
+        return (torch.arange(count, dtype=f32, device=device) + 0.5) * inv.to(device)
+
+    px = centres(size[0], X).view(1, 1, X)
+    py = centres(size[1], Y).view(1, Y, 1)
+    pz = centres(size[2], Z).view(Z, 1, 1)
+    s = torch.zeros((Z, Y, X), dtype=f32, device=device)
+    for b in np.asarray(balls, np.float32):
+        dx = px - float(b[0])
+        dy = py - float(b[1])
+        dz = pz - float(b[2])
+        d2 = (dx * dx + dy * dy) + dz * dz
+        # tensor / tensor (a python scalar over a tensor becomes a reciprocal
+        # times the scalar, rounded twice)
+        s = s + torch.tensor(float(b[3]), dtype=f32, device=device) / d2
+    return s.contiguous()
+
+
 # ---------------------------------------------------------------------------
 def upsampled_noise(size=256, coarse=32, seed=1):
     """Coarse ~U[0,1]^3 grid trilinearly upsampled, corner-aligned

@@ -209,6 +209,64 @@ int32_t HPMCcheckFieldSnippet(const char* source, uint32_t flags, char* log, siz
  * out = {tile_x, tile_y, tiles_x, tiles_y, size_l2}. */
 int32_t HPMCgetLayout(struct HPMCHistoPyramid* h, int32_t out[5]);
 
+/* ---- batched builds (no counterpart in the reference) -------------------
+ *
+ * Many FP32 volumes of uneven sizes built and extracted together: one
+ * classification launch, one two-pass scan and one persistent extraction
+ * launch cover every volume, and the chain can be captured once and replayed
+ * as a hipGraph.  Each volume's count and vertex list are those of a
+ * HistoPyramid of that volume alone (same codes, same order, same bits); the
+ * output is the concatenation of the volumes' lists in the order they were
+ * added.  For callers that extract many small or medium volumes per step
+ * (training loops, serving).  Errors: HPMC_* codes; the reason is in
+ * HPMCgetLastError(NULL) / HPMCgetLastErrorString(NULL) of the calling
+ * thread.  Work runs on the batch's stream (HPMCbatchSetStream, else a
+ * stream the batch creates that orders with the null stream). */
+struct HPMCBatch;
+
+struct HPMCBatch* HPMCcreateBatch(struct HPMCConstants* c);
+void HPMCdestroyBatch(struct HPMCBatch* b);
+
+/* Adds a volume (device FP32 lattice x*y*z, x fastest; grid = lattice - 1,
+ * extent as HPMCsetGridExtent).  Returns its index, or a negative HPMC_*
+ * code. */
+int32_t HPMCbatchAddVolume(struct HPMCBatch* b, const float* volume, int32_t x, int32_t y, int32_t z,
+                           float ext_x, float ext_y, float ext_z);
+int32_t HPMCbatchNumVolumes(struct HPMCBatch* b);
+
+/* Launch shapes; 0 picks automatically.  classify_waves: waves per
+ * classification workgroup (each covers 64 lattice columns x 16 cell rows);
+ * zseg: slices per classification workgroup (8, 16 or 32); emit_workgroups:
+ * persistent extraction workgroups.  Results do not depend on them. */
+int32_t HPMCbatchSetTuning(struct HPMCBatch* b, int32_t classify_waves, int32_t zseg, int32_t emit_workgroups);
+/* The shapes in use after the last configuration: {classify_waves, zseg,
+ * emit_workgroups}. */
+int32_t HPMCbatchGetTuning(struct HPMCBatch* b, int32_t out[3]);
+int32_t HPMCbatchSetStream(struct HPMCBatch* b, void* stream);
+
+/* Classifies and scans every volume (enqueued, no host wait). */
+int32_t HPMCbatchBuild(struct HPMCBatch* b, float threshold);
+/* Waits for the last build; per-volume counts and output bases (arrays of
+ * HPMCbatchNumVolumes entries, either may be NULL) and the total. */
+int32_t HPMCbatchGetCounts(struct HPMCBatch* b, uint32_t* counts, uint32_t* bases, uint64_t* total);
+/* Extracts every volume into `out` (N3F_V3F, capacity vertices) and, if not
+ * NULL, (cell, edge) records (uint32 pairs, cell index within its volume).
+ * Vertices past the capacity are dropped; after HPMCbatchGetCounts a too
+ * small buffer is an HPMC_ERR_CAPACITY error instead. */
+int32_t HPMCbatchExtract(struct HPMCBatch* b, float* out, uint64_t capacity, void* cell_edge);
+
+/* Captures build(threshold) + extract(out, capacity, cell_edge) into a
+ * hipGraph (replacing any earlier one); HPMCbatchLaunchGraph replays it.
+ * Counts of a replay: HPMCbatchGetCounts. */
+int32_t HPMCbatchCaptureGraph(struct HPMCBatch* b, float threshold, float* out, uint64_t capacity, void* cell_edge);
+int32_t HPMCbatchLaunchGraph(struct HPMCBatch* b);
+
+/* Autotuner: times build + extract into `out` under each candidate launch
+ * shape (repetitions runs each, median) and keeps the fastest; best_ms (may
+ * be NULL) receives its time.  Results are identical under every shape. */
+int32_t HPMCbatchAutotune(struct HPMCBatch* b, float threshold, float* out, uint64_t capacity, int32_t repetitions,
+                          double* best_ms);
+
 #ifdef __cplusplus
 }
 #endif
