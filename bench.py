@@ -13,6 +13,10 @@ carries verts/s, the whole-step HBM fraction for the algorithmic bytes
 B = 4 * lattice points + 24 * vertices, per-stage HIP-event times, the
 roofline of the dominant kernel and the CPU baseline (the oracle's C port of
 the same path, one thread, on this host).
+
+The line's "batched" object is the batched/plain comparison: 256 metaball
+volumes of uneven sizes (measure_batched), built and extracted as one
+HPMCBatch (autotuned, replayed as a hipGraph) versus 256 plain builds.
 """
 import argparse
 import json
@@ -64,6 +68,101 @@ def cpu_baseline(n, reps):
             "sample": "full %d^3 Cayley volume, %d runs (median %.2f s), %d vertices" % (n, reps, t, nv)}
 
 
+def measure_batched(reps, count=256, iso=1.0):
+    """Batched vs plain builds of `count` metaball volumes of uneven sizes
+    (each axis ~U{16..128}, 4-32 balls, seed 0), iso 1, positions + normals.
+
+    plain:        per volume, the reference's call order -- build, count (a
+                  host wait), extract into a buffer of that count's size;
+    plain_async:  per volume build + extract into pre-sized buffers, one
+                  wait at the end (the most favourable plain schedule);
+    batched:      one HPMCBatch: autotuned launch shapes, build + extract
+                  captured as a hipGraph, one replay + counts per step.
+    Every variant produces the same vertices (the GPU tests prove it)."""
+    import numpy as np
+    import torch
+    import hpmc_amd
+    from hpmc_amd import workloads
+    dev = torch.device("cuda", 0)
+    specs = workloads.batch_volume_specs(count=count, seed=0)
+    vols = [workloads.metaballs_torch(size, balls, dev) for size, balls in specs]
+    torch.cuda.synchronize()
+    points = sum(int(v.numel()) for v in vols)
+    cells = sum((s[0] - 1) * (s[1] - 1) * (s[2] - 1) for s, _ in specs)
+
+    # plain HistoPyramids, configured and warmed up outside the timing
+    isos = []
+    outs = []
+    for (size, _), v in zip(specs, vols):
+        h = hpmc_amd.IsoSurface(size)
+        h.set_volume(v)
+        h.build(iso)
+        n = h.count()
+        o = torch.empty((max(n, 1), 6), dtype=torch.float32, device=dev)
+        h.bind_output(o)
+        if n:
+            h.extract()
+        isos.append(h)
+        outs.append(o)
+    torch.cuda.synchronize()
+    plain_vertices = sum(h.count() for h in isos)
+
+    def plain_step():
+        for h in isos:
+            h.build(iso)
+            if h.count():
+                h.extract()
+        torch.cuda.synchronize()
+
+    def plain_async_step():
+        for h in isos:
+            h.build(iso)
+            h.extract()
+        torch.cuda.synchronize()
+
+    b = hpmc_amd.Batch()
+    for v in vols:
+        b.add_volume(v)
+    b.build(iso)
+    counts, bases, total = b.counts()
+    assert total == plain_vertices, (total, plain_vertices)
+    out = torch.empty((total, 6), dtype=torch.float32, device=dev)
+    tuning, tuned_ms = b.autotune(iso, out, repetitions=3)
+    b.capture_graph(iso, out)
+
+    def batched_step():
+        b.launch_graph()
+        b.counts()
+
+    def batched_eager_step():
+        b.build(iso)
+        b.extract(out)
+        b.counts()
+
+    def timeit(fn):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    res = {
+        "workload": "%d metaball FP32 volumes, axes U{16..128}, 4-32 balls, seed 0, iso %g" % (count, iso),
+        "volumes": count, "lattice_points": points, "cells": cells, "vertices": int(total),
+        "plain_ms": timeit(plain_step), "plain_async_ms": timeit(plain_async_step),
+        "batched_eager_ms": timeit(batched_eager_step), "batched_ms": timeit(batched_step),
+        "tuning": {"classify_waves": tuning[0], "zseg": tuning[1], "emit_workgroups": tuning[2],
+                   "autotuned_ms": tuned_ms},
+    }
+    res["speedup"] = res["plain_ms"] / res["batched_ms"]
+    res["speedup_vs_plain_async"] = res["plain_async_ms"] / res["batched_ms"]
+    res["batched_cells_per_s"] = cells / (res["batched_ms"] * 1e-3)
+    for h in isos:
+        h.close()
+    b.close()
+    return res
+
+
 def copy_calibration(steps):
     """A plain 512 MiB device copy (torch copy_), for scaling the HBM counters:
     each launch reads and writes exactly 536,870,912 bytes."""
@@ -99,6 +198,8 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched/plain comparison")
+    ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 512 MiB copy instead (counter calibration)")
     args = ap.parse_args()
@@ -184,10 +285,12 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic() if n == 512 else None, "algorithmic_bytes": lattice_bytes},
     }
+    iso.close()
+    if not args.no_batch:
+        line["batched"] = measure_batched(args.batch_reps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_reps)
     print(json.dumps(line))
-    iso.close()
 
 
 if __name__ == "__main__":
