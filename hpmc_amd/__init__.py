@@ -69,6 +69,8 @@ SIGNATURES = [
     ("HPMCgetStageTimes", _i32, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u32)]),
     ("HPMCcheckFieldSnippet", _i32, [ctypes.c_char_p, _u32, ctypes.c_char_p, ctypes.c_size_t]),
     ("HPMCgetLayout", _i32, [_vp, ctypes.POINTER(_i32)]),
+    ("HPMCadjoint", _i32, [_vp, _vp, _vp, _vp]),
+    ("HPMCbatchAdjoint", _i32, [_vp, _vp, _vp, _vp]),
     ("HPMCcreateBatch", _vp, [_vp]),
     ("HPMCdestroyBatch", None, [_vp]),
     ("HPMCbatchAddVolume", _i32, [_vp, _vp, _i32, _i32, _i32, _f, _f, _f]),
@@ -266,6 +268,18 @@ class IsoSurface:
             raise HPMCError("level4 failed: %s" % (last_error(self.h)[1],))
         return buf
 
+    def adjoint(self, grad_vertices):
+        """dL/dfield (shaped like the volume) and dL/dT (0-dim) of the last
+        build + extraction from dL/dvertices (count, 6); deterministic."""
+        import torch
+        assert self._volume is not None, "adjoint needs a volume field"
+        gf = torch.empty_like(self._volume)
+        gt = torch.empty((), dtype=torch.float32, device=self._volume.device)
+        gv = grad_vertices.contiguous() if grad_vertices is not None else None
+        _check(lib().HPMCadjoint(self._traversal(), ctypes.c_void_p(gv.data_ptr()) if gv is not None else None,
+                                 ctypes.c_void_p(gf.data_ptr()), ctypes.c_void_p(gt.data_ptr())), self.h)
+        return gf, gt
+
     def set_profiling(self, enable=True):
         _check(lib().HPMCsetProfiling(self.h, int(bool(enable))), self.h)
 
@@ -364,6 +378,24 @@ class Batch:
 
     def launch_graph(self):
         _check(lib().HPMCbatchLaunchGraph(self.b))
+
+    def adjoint(self, grad_vertices):
+        """(dL/dfield per volume, dL/dT) of the last build + extraction from
+        dL/dvertices (total, 6) of the concatenated output; deterministic."""
+        import torch
+        dev = self._volumes[0].device
+        n_lat = sum(v.numel() for v in self._volumes)
+        gf = torch.empty(n_lat, dtype=torch.float32, device=dev)
+        gt = torch.empty((), dtype=torch.float32, device=dev)
+        gv = grad_vertices.contiguous() if grad_vertices is not None else None
+        _check(lib().HPMCbatchAdjoint(self.b, ctypes.c_void_p(gv.data_ptr()) if gv is not None else None,
+                                      ctypes.c_void_p(gf.data_ptr()), ctypes.c_void_p(gt.data_ptr())))
+        grads = []
+        o = 0
+        for v in self._volumes:
+            grads.append(gf[o:o + v.numel()].view(v.shape))
+            o += v.numel()
+        return grads, gt
 
     def autotune(self, iso, out, repetitions=3):
         """Picks the fastest launch shapes for build + extract; returns (shape, ms)."""

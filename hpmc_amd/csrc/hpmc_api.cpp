@@ -21,6 +21,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "hpmc.h"
 #include "hpmc_device.h"
 
@@ -62,7 +64,136 @@ extern "C" __global__ __launch_bounds__(256) void hpmc_level4_gather(const hpmc:
     hpmc::level4_body(d, colcnt, out);
 }
 
+HPMC_DEFINE_ADJOINT_KERNEL(hpmc_adjoint_volume, HPMC_VOLUME_FIELD, false)
+HPMC_DEFINE_ADJOINT_KERNEL(hpmc_adjoint_volume_batch, HPMC_VOLUME_FIELD, true)
+
+extern "C" __global__ __launch_bounds__(256) void hpmc_adjoint_segments(const uint32_t* keys, const float* vals,
+                                                                        uint32_t n, float* grad)
+{
+    hpmc::adjoint_segments_body(keys, vals, n, grad);
+}
+
+extern "C" __global__ __launch_bounds__(256) void hpmc_sum_partials(const float* x, uint32_t n, float* part)
+{
+    hpmc::sum_partials_body(x, n, part);
+}
+
+extern "C" __global__ __launch_bounds__(256) void hpmc_sum_final(const float* part, float* out)
+{
+    hpmc::sum_final_body(part, out);
+}
+
+extern "C" __global__ __launch_bounds__(256) void hpmc_fill_f32(float* p, uint64_t n, float v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 #include "hpmc_plan.h"
+
+namespace hpmc_host {
+
+// Scratch of the adjoint, grown on demand: 8 (key, value) pairs per vertex
+// twice (sort input and output), per-vertex dL/dT terms, partial sums and
+// the radix sort's temporary storage.
+struct AdjointScratch {
+    uint64_t cap_pairs = 0;
+    uint32_t* keys[2] = {nullptr, nullptr};
+    float* vals[2] = {nullptr, nullptr};
+    float* dT = nullptr;
+    float* partials = nullptr;
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+};
+
+inline void adjoint_free(AdjointScratch& s)
+{
+    for (int i = 0; i < 2; ++i) {
+        if (s.keys[i]) (void)hipFree(s.keys[i]);
+        if (s.vals[i]) (void)hipFree(s.vals[i]);
+    }
+    if (s.dT) (void)hipFree(s.dT);
+    if (s.partials) (void)hipFree(s.partials);
+    if (s.sort_tmp) (void)hipFree(s.sort_tmp);
+    s = AdjointScratch();
+}
+
+#define ADJ_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess) {                                                            \
+            *err = std::string(#expr) + ": " + hipGetErrorString(_e);                      \
+            return HPMC_ERR_HIP;                                                           \
+        }                                                                                  \
+    } while (0)
+
+// dL/dfield (n_lattice floats, every sample written) and dL/dT (one float)
+// of the last extraction of plan p (nvert vertices at threshold T), from
+// dL/dvertices gout (nvert x 6, normal then position).  Deterministic:
+// per-vertex contributions, a stable sort by sample, in-order run sums.
+inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, const Kernel& k, float T, uint32_t nvert,
+                           const float* gout, float* grad_field, float* grad_iso, const uint64_t* cases,
+                           hipStream_t stream, std::string* err)
+{
+    const uint64_t npairs = 8ull * nvert;
+    if (npairs > 0xffffffffull) {
+        *err = "too many vertices for the adjoint";
+        return HPMC_ERR_SIZE;
+    }
+    if (npairs > s.cap_pairs || !s.partials) {
+        ADJ_TRY(hipStreamSynchronize(stream));
+        adjoint_free(s);
+        const uint64_t cap = std::max<uint64_t>(npairs, 8);
+        for (int i = 0; i < 2; ++i) {
+            ADJ_TRY(hipMalloc(&s.keys[i], cap * sizeof(uint32_t)));
+            ADJ_TRY(hipMalloc(&s.vals[i], cap * sizeof(float)));
+        }
+        ADJ_TRY(hipMalloc(&s.dT, cap / 8 * sizeof(float)));
+        ADJ_TRY(hipMalloc(&s.partials, hpmc::kReduceBlocks * sizeof(float)));
+        ADJ_TRY(rocprim::radix_sort_pairs(nullptr, s.sort_tmp_bytes, s.keys[0], s.keys[1], s.vals[0], s.vals[1],
+                                          (unsigned)cap, 0, 32, stream));
+        ADJ_TRY(hipMalloc(&s.sort_tmp, std::max<size_t>(s.sort_tmp_bytes, 16)));
+        s.cap_pairs = cap;
+    }
+    hipLaunchKernelGGL(hpmc_fill_f32, dim3(1024), dim3(256), 0, stream, grad_field, (uint64_t)p.n_lattice, 0.0f);
+    ADJ_TRY(hipGetLastError());
+    if (nvert == 0) {
+        hipLaunchKernelGGL(hpmc_fill_f32, dim3(1), dim3(64), 0, stream, grad_iso, (uint64_t)1, 0.0f);
+        ADJ_TRY(hipGetLastError());
+        return HPMC_OK;
+    }
+    {
+        const hpmc::VolDesc* vols = p.d_vols;
+        float Ta = T;
+        const uint2* it = p.items;
+        const uint32_t* lo = p.l4_off;
+        const hpmc::Totals* tt = p.d_totals;
+        uint32_t cnt = nvert;
+        const uint64_t* cs = cases;
+        const float* go = gout;
+        uint32_t* ko = s.keys[0];
+        float* vo = s.vals[0];
+        float* dt = s.dT;
+        void* args[] = {&vols, &Ta, &it, &lo, &tt, &cnt, &cs, &go, &ko, &vo, &dt};
+        ADJ_TRY(launch(k, (unsigned)p.emit_wgs, 256, args, stream));
+    }
+    // keys < n_lattice < 2^end_bit - 1, so the unused-pair key (all ones)
+    // sorts last on the low end_bit bits as well
+    int end_bit = 1;
+    while (end_bit < 32 && ((uint64_t)1 << end_bit) <= p.n_lattice) ++end_bit;
+    size_t tmp_bytes = s.sort_tmp_bytes;
+    ADJ_TRY(rocprim::radix_sort_pairs(s.sort_tmp, tmp_bytes, s.keys[0], s.keys[1], s.vals[0], s.vals[1],
+                                      (unsigned)npairs, 0, end_bit, stream));
+    hipLaunchKernelGGL(hpmc_adjoint_segments, dim3(2048), dim3(256), 0, stream, (const uint32_t*)s.keys[1],
+                       (const float*)s.vals[1], (uint32_t)npairs, grad_field);
+    hipLaunchKernelGGL(hpmc_sum_partials, dim3(hpmc::kReduceBlocks), dim3(256), 0, stream, (const float*)s.dT,
+                       nvert, s.partials);
+    hipLaunchKernelGGL(hpmc_sum_final, dim3(1), dim3(256), 0, stream, (const float*)s.partials, grad_iso);
+    ADJ_TRY(hipGetLastError());
+    return HPMC_OK;
+}
+
+}  // namespace hpmc_host
 
 namespace {
 
@@ -156,6 +287,7 @@ struct HPMCHistoPyramid {
     hpmc_host::Plan plan;
     hpmc_host::Kernel k_classify, k_emit, k_codes;
     hipEvent_t ev_totals = nullptr;  // after the scan: totals are in pinned memory
+    hpmc_host::AdjointScratch adjoint;
     bool built = false;
     bool count_valid = false;
     uint32_t count = 0;
@@ -221,6 +353,7 @@ void ok(HPMCHistoPyramid* h)
 void free_device(HPMCHistoPyramid* h)
 {
     hpmc_host::plan_free(h->plan);
+    hpmc_host::adjoint_free(h->adjoint);
     if (h->ev_totals) (void)hipEventDestroy(h->ev_totals);
     if (h->module) (void)hipModuleUnload(h->module);
     h->ev_totals = nullptr;
@@ -893,6 +1026,26 @@ int32_t HPMCcheckFieldSnippet(const char* source, uint32_t flags, char* log, siz
     return HPMC_OK;
 }
 
+int32_t HPMCadjoint(struct HPMCTraversalHandle* th, const float* grad_vertices, float* grad_field, float* grad_iso)
+{
+    if (!th || !th->h || !grad_field || !grad_iso) return fail(th ? th->h : nullptr, HPMC_ERR_NULL, "NULL argument");
+    HPMCHistoPyramid* h = th->h;
+    if (h->broken) return fail(h, HPMC_ERR_BROKEN, "HistoPyramid is broken");
+    if (h->tainted || !h->built) return fail(h, HPMC_ERR_ARG, "HistoPyramid has not been built since it was configured");
+    if (h->mode != kFieldVolume || h->gradient || h->binary_active)
+        return fail(h, HPMC_ERR_UNSUPPORTED, "the adjoint covers continuous FP32 volume fields");
+    if (!wait_count(h)) return h->last_error;
+    if (h->count && !grad_vertices) return fail(h, HPMC_ERR_NULL, "grad_vertices == NULL");
+    std::string err;
+    const int32_t rc = hpmc_host::adjoint_run(h->plan, h->adjoint,
+                                              hpmc_host::Kernel{(const void*)hpmc_adjoint_volume, nullptr},
+                                              h->threshold, h->count, grad_vertices, grad_field, grad_iso,
+                                              h->constants->d_cases, h->stream, &err);
+    if (rc != HPMC_OK) return fail(h, rc, "%s", err.c_str());
+    ok(h);
+    return HPMC_OK;
+}
+
 int32_t HPMCgetLayout(struct HPMCHistoPyramid* h, int32_t out[5])
 {
     if (!h || !out) return fail(h, HPMC_ERR_NULL, "NULL argument");
@@ -921,6 +1074,7 @@ struct HPMCBatch {
     bool built = false;
     bool counts_known = false;
     float last_threshold = 0.0f;
+    hpmc_host::AdjointScratch adjoint;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 };
@@ -1001,6 +1155,7 @@ void HPMCdestroyBatch(struct HPMCBatch* b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_drop_graph(b);
     hpmc_host::plan_free(b->plan);
+    hpmc_host::adjoint_free(b->adjoint);
     if (b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1148,6 +1303,23 @@ int32_t HPMCbatchLaunchGraph(struct HPMCBatch* b)
     BATCH_TRY(hipGraphLaunch(b->graph_exec, b->stream));
     b->built = true;
     b->counts_known = false;
+    ok(nullptr);
+    return HPMC_OK;
+}
+
+int32_t HPMCbatchAdjoint(struct HPMCBatch* b, const float* grad_vertices, float* grad_field, float* grad_iso)
+{
+    if (!b || !grad_field || !grad_iso) return fail(nullptr, HPMC_ERR_NULL, "NULL argument");
+    if (!b->built || b->tainted) return fail(nullptr, HPMC_ERR_ARG, "batch not built since it was configured");
+    BATCH_TRY(hipStreamSynchronize(b->stream));
+    const uint32_t n = b->plan.h_totals->vertices;
+    if (n && !grad_vertices) return fail(nullptr, HPMC_ERR_NULL, "grad_vertices == NULL");
+    std::string err;
+    const int32_t rc = hpmc_host::adjoint_run(b->plan, b->adjoint,
+                                              hpmc_host::Kernel{(const void*)hpmc_adjoint_volume_batch, nullptr},
+                                              b->last_threshold, n, grad_vertices, grad_field, grad_iso,
+                                              b->constants->d_cases, b->stream, &err);
+    if (rc != HPMC_OK) return fail(nullptr, rc, "%s", err.c_str());
     ok(nullptr);
     return HPMC_OK;
 }

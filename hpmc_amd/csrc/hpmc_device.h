@@ -88,6 +88,7 @@ struct VolDesc {
     uint32_t n_l4tiles;
     uint32_t col_tile0;       // first column scan tile (index among column tiles)
     uint32_t n_coltiles;
+    uint32_t lattice_base;    // first sample of the volume in a concatenated field (adjoint keys)
 };
 
 // Vertex base and count of one volume of a batch.
@@ -168,13 +169,15 @@ __device__ __forceinline__ void wave_lds_sync()
 struct VolumeField {
     const float* __restrict__ v;
     int sx, sy, sz;
-    __device__ __forceinline__ float at(int i, int j, int k) const
+    // linear index of the (clamped) sample a fetch at (i, j, k) reads
+    __device__ __forceinline__ uint32_t index(int i, int j, int k) const
     {
         i = min(max(i, 0), sx - 1);
         j = min(max(j, 0), sy - 1);
         k = min(max(k, 0), sz - 1);
-        return v[((size_t)k * sy + j) * sx + i];
+        return ((uint32_t)k * (uint32_t)sy + (uint32_t)j) * (uint32_t)sx + (uint32_t)i;
     }
+    __device__ __forceinline__ float at(int i, int j, int k) const { return v[index(i, j, k)]; }
     // Plane-wise access goes through a buffer resource per plane (a scalar
     // 128-bit descriptor) and 32-bit per-lane byte offsets.
     typedef __amdgpu_buffer_rsrc_t Plane;
@@ -715,12 +718,25 @@ constexpr int kPlaneLoads = (kBrickPlane + 63) / 64;   // 6 per lane per plane
 //   continuous: t from the two edge samples, normal from forward differences
 //   binary:     edge midpoint, normal from the per-case table (threshold 0.5)
 //   gradient:   t from the two samples' values, normal from their gradients
-enum EmitMode { kEmitContinuous = 0, kEmitBinary = 1, kEmitGradient = 2 };
+enum EmitMode { kEmitContinuous = 0, kEmitBinary = 1, kEmitGradient = 2, kEmitAdjoint = 3 };
+
+// Adjoint of continuous emission: per output vertex o, dL/d(vertex o) is
+// read from gout[6o..6o+5] (normal, position) and the vertex's
+// contributions to dL/dfield are written as 8 (key, value) pairs at
+// keys/vals[8o..8o+7] (key = lattice_base + sample index, kAdjointNoKey for
+// an unused pair) and its contribution to dL/dT at dT[o].
+constexpr uint32_t kAdjointNoKey = 0xffffffffu;
+struct AdjointIO {
+    const float* gout;
+    uint32_t* keys;
+    float* vals;
+    float* dT;
+};
 
 // Lattice planes an emit item stages: z, z+1 for the corners, z+2 for the
 // forward differences of continuous mode.
 template <int MODE>
-constexpr int emit_planes() { return MODE == kEmitContinuous ? 3 : 2; }
+constexpr int emit_planes() { return MODE == kEmitContinuous || MODE == kEmitAdjoint ? 3 : 2; }
 
 struct EmitWaveShared {
     float plane[3][kBrickPlane];   // ring: plane p in slot p % 3
@@ -808,7 +824,8 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                                           const uint2* __restrict__ items, const uint32_t* __restrict__ l4_off,
                                           const Totals* __restrict__ totals,
                                           float* __restrict__ out, CellEdge* __restrict__ out_ce,
-                                          uint32_t capacity, const float4* __restrict__ bnorm, EmitShared& sh)
+                                          uint32_t capacity, const float4* __restrict__ bnorm, EmitShared& sh,
+                                          const AdjointIO adj = AdjointIO{})
 {
     constexpr int NP = emit_planes<MODE>();
     const int lane = threadIdx.x & 63;
@@ -963,6 +980,56 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                 // hardware reciprocal (1 ulp) keeps t within a few ulp
                 tt = (va - T) * __builtin_amdgcn_rcpf(va - vb);
                 const float u = 1.0f - tt;
+                if constexpr (MODE == kEmitAdjoint) {
+                    // p = sc (a + t axis), n = sc (T - (na (1-t) + nb t)),
+                    // t = (va - T) / (va - vb): chain rule back to the
+                    // samples va, vb, na_q = f(a + e_q), nb_q = f(b + e_q)
+                    // (na_axis is vb's sample) and to T
+                    const uint32_t o = base + v;
+                    if (o >= capacity) continue;
+                    const float2* gi = (const float2*)(adj.gout + (size_t)o * 6);
+                    const float2 g01 = gi[0], g23 = gi[1], g45 = gi[2];
+                    const float wx = g01.x * g.scx, wy = g01.y * g.scy, wz = g23.x * g.scz;
+                    float gt = axis == 0 ? g23.y * g.scx : (axis == 1 ? g45.x * g.scy : g45.y * g.scz);
+                    gt += wx * (nax - nbx_) + wy * (nay - nby_) + wz * (naz - nbz_);
+                    const float rD = __builtin_amdgcn_rcpf(va - vb);
+                    const float dva = gt * (T - vb) * rD * rD;
+                    const float dvb = gt * (va - T) * rD * rD;
+                    const float dT = (wx + wy + wz) - gt * rD;
+                    const float dnax = -wx * u, dnay = -wy * u, dnaz = -wz * u;
+                    const auto field = mk(dc);
+                    const int Ax = cx0 + ax, Ay = cy0 + ay, Az = cz + az;
+                    const int Bx = cx0 + bx_, By = cy0 + by_, Bz = cz + bz_;
+                    const uint32_t lb = dc.lattice_base;
+                    uint4 k0, k1;
+                    float4 v0, v1;
+                    k0.x = lb + field.index(Ax, Ay, Az);
+                    v0.x = dva;
+                    k0.y = lb + field.index(Bx, By, Bz);
+                    v0.y = dvb + (axis == 0 ? dnax : (axis == 1 ? dnay : dnaz));
+                    // the two non-axis forward samples of a
+                    const int q0 = axis == 0 ? 1 : 0, q1 = axis == 2 ? 1 : 2;
+                    k0.z = lb + field.index(Ax + (q0 == 0), Ay + (q0 == 1), Az);
+                    v0.z = q0 == 0 ? dnax : dnay;
+                    k0.w = lb + field.index(Ax, Ay + (q1 == 1), Az + (q1 == 2));
+                    v0.w = q1 == 1 ? dnay : dnaz;
+                    k1.x = lb + field.index(Bx + 1, By, Bz);
+                    v1.x = -wx * tt;
+                    k1.y = lb + field.index(Bx, By + 1, Bz);
+                    v1.y = -wy * tt;
+                    k1.z = lb + field.index(Bx, By, Bz + 1);
+                    v1.z = -wz * tt;
+                    k1.w = kAdjointNoKey;
+                    v1.w = 0.0f;
+                    uint4* kd = (uint4*)(adj.keys + (size_t)o * 8);
+                    float4* vd = (float4*)(adj.vals + (size_t)o * 8);
+                    kd[0] = k0;
+                    kd[1] = k1;
+                    vd[0] = v0;
+                    vd[1] = v1;
+                    adj.dT[o] = dT;
+                    continue;
+                }
                 nx = T - (nax * u + nbx_ * tt);
                 ny = T - (nay * u + nby_ * tt);
                 nz = T - (naz * u + nbz_ * tt);
@@ -986,6 +1053,59 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
         cur = nxt;
         nxt = after;
     }
+}
+
+// Adjoint reduction, after the (key, value) pairs are sorted by key with
+// ties in vertex order: each run of equal keys is summed in that order by
+// the thread at its first element (runs are short: a sample feeds at most a
+// few dozen vertices), so the result does not depend on the launch.
+__device__ __forceinline__ void adjoint_segments_body(const uint32_t* __restrict__ keys,
+                                                      const float* __restrict__ vals, uint32_t n,
+                                                      float* __restrict__ grad)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t k = keys[i];
+        if (k == kAdjointNoKey || (i > 0 && keys[i - 1] == k)) continue;
+        float s = 0.0f;
+        for (uint32_t j = i; j < n && keys[j] == k; ++j) s += vals[j];
+        grad[k] = s;
+    }
+}
+
+// dL/dT: a fixed-shape reduction of the per-vertex terms (partials of
+// kReduceBlocks contiguous ranges, then one block over the partials), the
+// same for every launch.
+constexpr int kReduceBlocks = 256;
+__device__ __forceinline__ float block_sum_f(float v, float* sh)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    float r = 0.0f;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ void sum_partials_body(const float* __restrict__ x, uint32_t n, float* __restrict__ part)
+{
+    __shared__ float sh[16];
+    const uint32_t per = (n + kReduceBlocks - 1) / kReduceBlocks;
+    const uint32_t lo = blockIdx.x * per, hi = min(lo + per, n);
+    float s = 0.0f;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) s += x[i];
+    s = block_sum_f(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__device__ __forceinline__ void sum_final_body(const float* __restrict__ part, float* __restrict__ out)
+{
+    __shared__ float sh[16];
+    float s = 0.0f;
+    for (int i = threadIdx.x; i < kReduceBlocks; i += blockDim.x) s += part[i];
+    s = block_sum_f(s, sh);
+    if (threadIdx.x == 0) *out = s;
 }
 
 // The level-4 counts in Morton key order (parity and debugging).
@@ -1057,4 +1177,21 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
         auto mk = [](const hpmc::VolDesc& dd) { return MAKE_FIELD; };                         \
         hpmc::emit_body<MODE, BATCH>(vols, mk, T, items, l4_off, totals, out, out_ce,         \
                                      capacity, bnorm, sh);                                    \
+    }
+
+// Adjoint of continuous emission (kEmitAdjoint) over the same items.
+#define HPMC_DEFINE_ADJOINT_KERNEL(NAME, MAKE_FIELD, BATCH)                                    \
+    extern "C" __global__ __launch_bounds__(256) void NAME(                                   \
+        const hpmc::VolDesc* vols, float T, const uint2* items, const uint32_t* l4_off,        \
+        const hpmc::Totals* totals, uint32_t count, const uint64_t* cases, const float* gout,  \
+        uint32_t* keys, float* vals, float* dT)                                               \
+    {                                                                                         \
+        __shared__ hpmc::EmitShared sh;                                                       \
+        sh.cases[threadIdx.x] = cases[threadIdx.x];                                           \
+        sh.lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);                           \
+        __syncthreads();                                                                      \
+        auto mk = [](const hpmc::VolDesc& dd) { return MAKE_FIELD; };                         \
+        hpmc::emit_body<hpmc::kEmitAdjoint, BATCH>(vols, mk, T, items, l4_off, totals, nullptr, \
+                                                   nullptr, count, nullptr, sh,               \
+                                                   hpmc::AdjointIO{gout, keys, vals, dT});    \
     }

@@ -150,6 +150,7 @@ struct Plan {
     uint32_t n_wg = 0;            // logical classification workgroups
     uint32_t n_l4t = 0, n_colt = 0;
     uint64_t n_cc = 0, n_l4 = 0, n_items_max = 0;
+    uint64_t n_lattice = 0;       // samples of all volumes (adjoint keys)
     int emit_wgs = 0;
     hpmc::VolDesc* d_vols = nullptr;
     uint32_t* d_cls_vol = nullptr;
@@ -260,6 +261,7 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
         d.n_l4tiles = (d.n_level4 + hpmc::kScanTile - 1) / hpmc::kScanTile;
         d.col_tile0 = (uint32_t)tile_vol_col.size();
         d.n_coltiles = (d.n_cc + hpmc::kScanTile - 1) / hpmc::kScanTile;
+        d.lattice_base = (uint32_t)p.n_lattice;
         const uint64_t wgs = (uint64_t)classify_bands(d.g, p.cls_waves) * d.g.npz;
         if (p.n_l4 + d.n_level4 > 0xffffffffull || p.n_wg + wgs > 0xffffffffull) {
             *err = "build too large for 32-bit offsets";
@@ -272,6 +274,12 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
         p.n_cc += d.n_cc;
         p.n_l4 += d.n_level4;
         p.n_wg += (uint32_t)wgs;
+        p.n_lattice += (uint64_t)d.g.sx * d.g.sy * d.g.sz;
+        if (p.n_lattice >= 0xffffffffull) {
+            *err = "build too large for 32-bit sample keys";
+            p = Plan();
+            return HPMC_ERR_SIZE;
+        }
         // a block holds at most 256 * 15 vertices: 15 work items per block slice
         p.n_items_max += (uint64_t)d.g.nbx * d.g.nby * d.g.cz * 15;
     }

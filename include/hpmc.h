@@ -205,6 +205,16 @@ int32_t HPMCgetStageTimes(struct HPMCHistoPyramid* h, double* ms, uint32_t* laun
 #define HPMC_SNIPPET_BINARY 2u
 int32_t HPMCcheckFieldSnippet(const char* source, uint32_t flags, char* log, size_t log_bytes);
 
+/* Adjoint of the last extraction (no counterpart in the reference): from
+ * dL/dvertices (device, count x 6 FP32 in the N3F_V3F order of the output)
+ * to dL/dfield (device, one FP32 per lattice sample, every sample written)
+ * and dL/dT (device, one FP32), through the edge interpolation
+ * t = (f(a)-T)/(f(a)-f(b)), the positions and the forward-difference
+ * normals.  Interpolation weights are recomputed, not stored; the result is
+ * deterministic (per-vertex contributions, stable sort by sample, in-order
+ * sums).  Continuous FP32 volume fields only; waits for the count. */
+int32_t HPMCadjoint(struct HPMCTraversalHandle* th, const float* grad_vertices, float* grad_field, float* grad_iso);
+
 /* Layout of the reference base level (hpmc/src/init.cpp:105-130):
  * out = {tile_x, tile_y, tiles_x, tiles_y, size_l2}. */
 int32_t HPMCgetLayout(struct HPMCHistoPyramid* h, int32_t out[5]);
@@ -260,6 +270,10 @@ int32_t HPMCbatchExtract(struct HPMCBatch* b, float* out, uint64_t capacity, voi
  * Counts of a replay: HPMCbatchGetCounts. */
 int32_t HPMCbatchCaptureGraph(struct HPMCBatch* b, float threshold, float* out, uint64_t capacity, void* cell_edge);
 int32_t HPMCbatchLaunchGraph(struct HPMCBatch* b);
+
+/* Adjoint of the last batched build + extraction (as HPMCadjoint):
+ * grad_field holds the volumes' lattices concatenated in order. */
+int32_t HPMCbatchAdjoint(struct HPMCBatch* b, const float* grad_vertices, float* grad_field, float* grad_iso);
 
 /* Autotuner: times build + extract into `out` under each candidate launch
  * shape (repetitions runs each, median) and keeps the fastest; best_ms (may

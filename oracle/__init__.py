@@ -24,6 +24,9 @@ own formulas; the tests that do it are named in brackets):
   emit_binary     midpoints are exact; normals as above
   emit_gradient   a linear field with its exact gradient gives exact t,
                   positions and normals T - grad (test_emit_gradient_*)
+  adjoint         central finite differences (h = 1e-3) of emit_f64_field
+                  on samples and threshold away from sign changes
+                  (test_adjoint_*)
 """
 import ctypes
 import json
@@ -81,6 +84,9 @@ def lib():
         L.hpo_emit_f32.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, ctypes.c_float, vp, vp, vp, i64, vp]
         L.hpo_run_volume.restype = i64
         L.hpo_run_volume.argtypes = [vp, vp, vp, vp, ctypes.c_float, ctypes.c_int32, ctypes.c_int32, vp, i64]
+        L.hpo_emit_f64d.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, ctypes.c_double, vp, vp, vp, i64, vp]
+        L.hpo_adjoint_f64.restype = ctypes.c_double
+        L.hpo_adjoint_f64.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, ctypes.c_double, vp, vp, vp, i64, vp, vp]
         L.hpo_binary_normals.argtypes = [vp]
         L.hpo_emit_binary_f64.argtypes = [vp, vp, vp, vp, vp, i64, vp]
         L.hpo_emit_grad_f64.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, ctypes.c_float, vp, vp, vp, i64, vp, vp,
@@ -168,6 +174,41 @@ def emit(field, codes, cell, slot, T, ext=(1.0, 1.0, 1.0), cells=None, size=None
     lib().hpo_emit_f32(_ptr(field), _ptr(dims), _ptr(size), int(clamp), _ptr(cells), _ptr(ext), float(T),
                        _ptr(codes), _ptr(cell), _ptr(slot), n, _ptr(out))
     return out[:n]
+
+
+def emit_f64_field(field64, codes, cell, slot, T, ext=(1.0, 1.0, 1.0), cells=None, size=None, clamp=True):
+    """emit() on an FP64 field and FP64 threshold (finite-difference checks)."""
+    field64 = np.ascontiguousarray(field64, np.float64)
+    dims = np.array(field64.shape[::-1], np.int32)
+    size = _i32(size if size is not None else dims)
+    cells = _i32(cells if cells is not None else size - 1)
+    codes = np.ascontiguousarray(codes, np.uint8)
+    cell = np.ascontiguousarray(cell, np.uint32)
+    slot = np.ascontiguousarray(slot, np.uint8)
+    n = len(cell)
+    out = np.zeros((max(n, 1), 6), np.float64)
+    lib().hpo_emit_f64d(_ptr(field64), _ptr(dims), _ptr(size), int(clamp), _ptr(cells),
+                        _ptr(np.ascontiguousarray(ext, np.float32)), float(T), _ptr(codes), _ptr(cell), _ptr(slot),
+                        n, _ptr(out))
+    return out[:n]
+
+
+def adjoint(field, codes, cell, slot, T, gout, ext=(1.0, 1.0, 1.0), cells=None, size=None, clamp=True):
+    """FP64 adjoint of emit(): (dL/dfield shaped like field, dL/dT) for
+    dL/dvertices gout (N, 6)."""
+    field64 = np.ascontiguousarray(field, np.float64)
+    dims = np.array(field64.shape[::-1], np.int32)
+    size = _i32(size if size is not None else dims)
+    cells = _i32(cells if cells is not None else size - 1)
+    gout = np.ascontiguousarray(gout, np.float64)
+    codes = np.ascontiguousarray(codes, np.uint8)
+    cell = np.ascontiguousarray(cell, np.uint32)
+    slot = np.ascontiguousarray(slot, np.uint8)
+    dfield = np.zeros_like(field64)
+    dT = lib().hpo_adjoint_f64(_ptr(field64), _ptr(dims), _ptr(size), int(clamp), _ptr(cells),
+                               _ptr(np.ascontiguousarray(ext, np.float32)), float(T), _ptr(codes), _ptr(cell),
+                               _ptr(slot), len(cell), _ptr(gout), _ptr(dfield))
+    return dfield, float(dT)
 
 
 def binary_normals():

@@ -74,8 +74,8 @@ int hpo_layout(const int32_t size[3], const int32_t cells[3], int32_t out[5])
 }
 
 /* ------------------------------------------------------------------------ */
-static inline float sample(const float* f, const int32_t dims[3], const int32_t size[3],
-                           int clamp, int64_t i, int64_t j, int64_t k)
+static inline int64_t sample_index(const int32_t dims[3], const int32_t size[3], int clamp,
+                                   int64_t i, int64_t j, int64_t k)
 {
     if (clamp) {
         if (i > size[0] - 1) i = size[0] - 1;
@@ -85,7 +85,13 @@ static inline float sample(const float* f, const int32_t dims[3], const int32_t 
         if (j < 0) j = 0;
         if (k < 0) k = 0;
     }
-    return f[(k * dims[1] + j) * dims[0] + i];
+    return (k * dims[1] + j) * dims[0] + i;
+}
+
+static inline float sample(const float* f, const int32_t dims[3], const int32_t size[3],
+                           int clamp, int64_t i, int64_t j, int64_t k)
+{
+    return f[sample_index(dims, size, clamp, i, j, k)];
 }
 
 /* Base-level classification: hpmc/src/shadergen.cpp:86-171.  A cell's code  */
@@ -277,10 +283,34 @@ void hpo_level4(const uint8_t* counts, const int32_t cells[3], const int32_t lay
 /* The reference's texel-centre offset and rescale (556-563) reduce to this. */
 /* out6 = (nx, ny, nz, px, py, pz) per vertex (N3F_V3F, transform_feedback.cpp:292-296). */
 /* Double precision throughout, on the FP32 samples.                         */
+static void emit_f64_any(const float* ff, const double* fd, const int32_t dims[3], const int32_t size[3],
+                        int clamp, const int32_t cells[3], const float ext[3], double T,
+                        const uint8_t* codes, const uint32_t* cell, const uint8_t* slot, int64_t n,
+                        double* out6, double* out_t, double* out_mag);
+
 void hpo_emit_f64(const float* f, const int32_t dims[3], const int32_t size[3], int clamp,
                   const int32_t cells[3], const float ext[3], float T,
                   const uint8_t* codes, const uint32_t* cell, const uint8_t* slot, int64_t n,
                   double* out6, double* out_t, double* out_mag)
+{
+    emit_f64_any(f, NULL, dims, size, clamp, cells, ext, (double)T, codes, cell, slot, n, out6, out_t, out_mag);
+}
+
+/* The same on an FP64 field and threshold (finite-difference checks). */
+void hpo_emit_f64d(const double* f, const int32_t dims[3], const int32_t size[3], int clamp,
+                   const int32_t cells[3], const float ext[3], double T,
+                   const uint8_t* codes, const uint32_t* cell, const uint8_t* slot, int64_t n, double* out6)
+{
+    emit_f64_any(NULL, f, dims, size, clamp, cells, ext, T, codes, cell, slot, n, out6, NULL, NULL);
+}
+
+#define FS(i, j, k) (ff ? (double)ff[sample_index(dims, size, clamp, (i), (j), (k))] \
+                        : fd[sample_index(dims, size, clamp, (i), (j), (k))])
+
+static void emit_f64_any(const float* ff, const double* fd, const int32_t dims[3], const int32_t size[3],
+                        int clamp, const int32_t cells[3], const float ext[3], double T,
+                        const uint8_t* codes, const uint32_t* cell, const uint8_t* slot, int64_t n,
+                        double* out6, double* out_t, double* out_mag)
 {
     double sc[3];
     for (int a = 0; a < 3; ++a) sc[a] = (double)ext[a] / (double)cells[a];
@@ -292,25 +322,25 @@ void hpo_emit_f64(const float* f, const int32_t dims[3], const int32_t size[3], 
         int ax = e[3];
         int64_t b[3] = {a[0], a[1], a[2]};
         b[ax] += 1;
-        double va = sample(f, dims, size, clamp, a[0], a[1], a[2]);
-        double vb = sample(f, dims, size, clamp, b[0], b[1], b[2]);
+        double va = FS(a[0], a[1], a[2]);
+        double vb = FS(b[0], b[1], b[2]);
         double na[3], nb[3];
         for (int q = 0; q < 3; ++q) {
             int64_t d[3] = {0, 0, 0};
             d[q] = 1;
-            na[q] = sample(f, dims, size, clamp, a[0] + d[0], a[1] + d[1], a[2] + d[2]);
-            nb[q] = sample(f, dims, size, clamp, b[0] + d[0], b[1] + d[1], b[2] + d[2]);
+            na[q] = FS(a[0] + d[0], a[1] + d[1], a[2] + d[2]);
+            nb[q] = FS(b[0] + d[0], b[1] + d[1], b[2] + d[2]);
         }
-        double t = (va - (double)T) / (va - vb);
+        double t = (va - T) / (va - vb);
         double* o = out6 + 6 * v;
         for (int q = 0; q < 3; ++q) {
-            o[q] = ((double)T - (na[q] * (1.0 - t) + nb[q] * t)) * sc[q];
+            o[q] = (T - (na[q] * (1.0 - t) + nb[q] * t)) * sc[q];
             o[3 + q] = ((double)a[q] + (q == ax ? t : 0.0)) * sc[q];
         }
         if (out_t) out_t[v] = t;
         if (out_mag) {
             /* magnitude of the terms the normal is formed from (error scale) */
-            double m = fabs((double)T);
+            double m = fabs(T);
             for (int q = 0; q < 3; ++q) m = fmax(m, fmax(fabs(na[q]), fabs(nb[q])));
             out_mag[v] = m;
         }
@@ -350,6 +380,61 @@ void hpo_emit_f32(const float* f, const int32_t dims[3], const int32_t size[3], 
             o[3 + q] = ((float)a[q] + (q == ax ? t : 0.0f)) * sc[q];
         }
     }
+}
+
+#undef FS
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint of hpo_emit_f64 (no reference counterpart): given dL/dout6 per    */
+/* vertex (normal then position), accumulates dL/df into dfield (the field's */
+/* array, zeroed by the caller; a clamped sample credits the sample it       */
+/* reads) and returns dL/dT.  With D = va - vb, t = (va - T)/D:              */
+/*   dt/dva = (T - vb)/D^2, dt/dvb = (va - T)/D^2, dt/dT = -1/D;             */
+/*   p_q = sc_q (a_q + t [q = axis]),  n_q = sc_q (T - na_q (1-t) - nb_q t). */
+/* FP64 throughout.                                                         */
+double hpo_adjoint_f64(const double* f, const int32_t dims[3], const int32_t size[3], int clamp,
+                       const int32_t cells[3], const float ext[3], double T,
+                       const uint8_t* codes, const uint32_t* cell, const uint8_t* slot, int64_t n,
+                       const double* gout, double* dfield)
+{
+    double sc[3], dT = 0.0;
+    for (int a = 0; a < 3; ++a) sc[a] = (double)ext[a] / (double)cells[a];
+    for (int64_t v = 0; v < n; ++v) {
+        uint32_t c = cell[v];
+        int64_t x = c % cells[0], y = (c / cells[0]) % cells[1], z = c / ((int64_t)cells[0] * cells[1]);
+        const int* e = g_slot[codes[c]][slot[v]];
+        int64_t a[3] = {x + e[0], y + e[1], z + e[2]};
+        int ax = e[3];
+        int64_t b[3] = {a[0], a[1], a[2]};
+        b[ax] += 1;
+        int64_t ia = sample_index(dims, size, clamp, a[0], a[1], a[2]);
+        int64_t ib = sample_index(dims, size, clamp, b[0], b[1], b[2]);
+        int64_t ina[3], inb[3];
+        double na[3], nb[3];
+        for (int q = 0; q < 3; ++q) {
+            int64_t d[3] = {0, 0, 0};
+            d[q] = 1;
+            ina[q] = sample_index(dims, size, clamp, a[0] + d[0], a[1] + d[1], a[2] + d[2]);
+            inb[q] = sample_index(dims, size, clamp, b[0] + d[0], b[1] + d[1], b[2] + d[2]);
+            na[q] = f[ina[q]];
+            nb[q] = f[inb[q]];
+        }
+        double va = f[ia], vb = f[ib];
+        double D = va - vb, t = (va - T) / D;
+        const double* g = gout + 6 * v;
+        double gt = g[3 + ax] * sc[ax];
+        for (int q = 0; q < 3; ++q) {
+            double w = g[q] * sc[q];
+            dT += w;
+            gt += w * (na[q] - nb[q]);
+            dfield[ina[q]] += -w * (1.0 - t);
+            dfield[inb[q]] += -w * t;
+        }
+        dfield[ia] += gt * (T - vb) / (D * D);
+        dfield[ib] += gt * (va - T) / (D * D);
+        dT += -gt / D;
+    }
+    return dT;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -24,6 +24,13 @@ void hpo_emit_f32(const float* f, const int32_t dims[3], const int32_t size[3], 
                   const int32_t cells[3], const float ext[3], float T,
                   const uint8_t* codes, const uint32_t* cell, const uint8_t* slot, int64_t n,
                   float* out6);
+void hpo_emit_f64d(const double* f, const int32_t dims[3], const int32_t size[3], int clamp,
+                   const int32_t cells[3], const float ext[3], double T,
+                   const uint8_t* codes, const uint32_t* cell, const uint8_t* slot, int64_t n, double* out6);
+double hpo_adjoint_f64(const double* f, const int32_t dims[3], const int32_t size[3], int clamp,
+                       const int32_t cells[3], const float ext[3], double T,
+                       const uint8_t* codes, const uint32_t* cell, const uint8_t* slot, int64_t n,
+                       const double* gout, double* dfield);
 void hpo_binary_normals(float* out);
 void hpo_emit_binary_f64(const int32_t cells[3], const float ext[3], const uint8_t* codes,
                          const uint32_t* cell, const uint8_t* slot, int64_t n, double* out6);

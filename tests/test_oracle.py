@@ -283,3 +283,77 @@ def test_emit_gradient_linear_field_exact():
     p = out[:, 3:] / sc
     np.testing.assert_allclose(0.5 * p[:, 0] - 0.25 * p[:, 1] + 0.125 * p[:, 2], np.float32(T), atol=1e-6)
     np.testing.assert_allclose(out[:, :3], np.broadcast_to((np.float32(T) - np.array([0.5, -0.25, 0.125])) * sc, out[:, :3].shape), atol=1e-12)
+
+
+# --------------------------------------------------------------- adjoint ---
+def _sphere_noise(n, seed, sigma=0.01):
+    p = (np.arange(n) + 0.5) / n
+    d = np.sqrt((p[None, None, :] - 0.5) ** 2 + (p[None, :, None] - 0.5) ** 2 + (p[:, None, None] - 0.5) ** 2)
+    rng = np.random.default_rng(seed)
+    return (d - 0.3 + sigma * rng.standard_normal(d.shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize("clamp_edge", [False, True])
+def test_adjoint_matches_finite_differences(clamp_edge):
+    """dL/dfield and dL/dT of L = sum(gout * emit(f, T)) against central
+    differences (h = 1e-3) of the FP64 forward, on samples whose perturbation
+    keeps every sign (fixed topology)."""
+    n = 14
+    f = _sphere_noise(n, 5).astype(np.float64)
+    if clamp_edge:
+        # surface through the upper faces: forward differences clamp there
+        f = f - 0.18
+    T = 0.0
+    codes, counts = oracle.classify(f.astype(np.float32), T)
+    cells = (n - 1,) * 3
+    lay = oracle.layout((n,) * 3, cells)
+    cell, slot = oracle.vertex_list(counts, cells, lay)
+    assert len(cell) > 50
+    ext = (1.0, 2.0, 0.5)
+    rng = np.random.default_rng(1)
+    gout = rng.standard_normal((len(cell), 6))
+    dfield, dT = oracle.adjoint(f, codes, cell, slot, T, gout, ext=ext)
+
+    def loss(ff, TT):
+        return float((oracle.emit_f64_field(ff, codes, cell, slot, TT, ext=ext) * gout).sum())
+
+    h = 1e-3
+
+    def fd_sample(idx, step):
+        fp = f.copy()
+        fp[idx] += step
+        fm = f.copy()
+        fm[idx] -= step
+        return (loss(fp, T) - loss(fm, T)) / (2 * step)
+
+    cand = np.argwhere((np.abs(f - T) > 4 * h) & (dfield != 0))
+    assert len(cand) > 20
+    picks = cand[rng.choice(len(cand), 40, replace=False)]
+    for idx in map(tuple, picks):
+        # central differences at h = 1e-3 with one Richardson step (h, h/2):
+        # t = (va - T)/(va - vb) is rational, so plain O(h^2) differences
+        # are only good to (h/D)^2 for edge differences D of a few 1e-2
+        fd = (4.0 * fd_sample(idx, h / 2) - fd_sample(idx, h)) / 3.0
+        assert abs(fd - dfield[idx]) <= 2e-6 * max(1.0, abs(fd)), (idx, fd, dfield[idx])
+    # untouched samples get exactly zero
+    assert (dfield[np.abs(f - T) > 0.2] == 0).sum() > 0
+
+
+def test_adjoint_dT_finite_difference():
+    n = 12
+    f = _sphere_noise(n, 9).astype(np.float64)
+    T = 0.013
+    assert np.min(np.abs(f - T)) > 1e-4
+    h = min(1e-3, np.min(np.abs(f - T)) / 4)
+    codes, counts = oracle.classify(f.astype(np.float32), T)
+    cells = (n - 1,) * 3
+    cell, slot = oracle.vertex_list(counts, cells, oracle.layout((n,) * 3, cells))
+    gout = np.random.default_rng(2).standard_normal((len(cell), 6))
+    _, dT = oracle.adjoint(f, codes, cell, slot, T, gout)
+    def fd_T(step):
+        lp = float((oracle.emit_f64_field(f, codes, cell, slot, T + step) * gout).sum())
+        lm = float((oracle.emit_f64_field(f, codes, cell, slot, T - step) * gout).sum())
+        return (lp - lm) / (2 * step)
+
+    fd = (4.0 * fd_T(h / 2) - fd_T(h)) / 3.0
+    assert abs(fd - dT) <= 2e-6 * max(1.0, abs(fd)), (fd, dT)
