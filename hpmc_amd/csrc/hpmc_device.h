@@ -155,6 +155,29 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Inclusive wave-wide scans over 64 lanes with DPP row shifts and row
+// broadcasts (no LDS round trips): row_shr 1, 2, 4, 8 within each row of 16
+// lanes, then row_bcast 15 and 31 carry row totals forward.  Inactive or
+// out-of-row sources contribute the identity 0 (inputs are non-negative).
+template <bool MAX>
+__device__ __forceinline__ uint32_t wave_scan_u32(uint32_t x)
+{
+    auto op = [](uint32_t a, uint32_t b) { return MAX ? (a > b ? a : b) : a + b; };
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false));   // row_shr:1
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false));   // row_shr:2
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false));   // row_shr:4
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false));   // row_shr:8
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false));   // row_bcast:15
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false));   // row_bcast:31
+    return x;
+}
+
+// The value of lane - 1 (0 in lane 0): DPP wave_shr:1.
+__device__ __forceinline__ uint32_t wave_shr1_u32(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x138, 0xf, 0xf, false);
+}
+
 // ---------------------------------------------------------------------------
 // Field sources.  at(i,j,k) returns the sample at lattice index (i,j,k).
 
@@ -738,13 +761,16 @@ struct AdjointIO {
 template <int MODE>
 constexpr int emit_planes() { return MODE == kEmitContinuous || MODE == kEmitAdjoint ? 3 : 2; }
 
+// A staged plane holds the 18x18 window in kPlaneStride slots (6 loads of 64
+// lanes, no tail); planes sit kPlaneStride floats apart so a sample and its
+// +z neighbour are one ds_read2st64 apart.
+constexpr int kPlaneStride = 64 * kPlaneLoads;   // 384
+
 struct EmitWaveShared {
-    float plane[3][kBrickPlane];   // ring: plane p in slot p % 3
-    uint16_t l0[256];              // per-cell vertex counts (Morton order)
-    uint16_t l1[64];               // per-quad sums
-    uint16_t l2[16];
-    uint16_t l3[4];
-    uint8_t code[256];
+    float plane[3][kPlaneStride];  // planes z, z+1, z+2 of the item
+    uint16_t cexc[256];            // per cell (Morton order): first vertex in the block
+    uint8_t code[256];             // per cell: HPMC code
+    uint8_t vcell[kEmitChunk + 64];  // per vertex of the item's chunk: its cell (+ a spare byte per lane)
 };
 
 struct EmitShared {
@@ -752,20 +778,6 @@ struct EmitShared {
     uint8_t lut[256];
     EmitWaveShared w[kEmitWaves];
 };
-
-// One level of the in-block descent.  A node holds the inclusive prefix
-// sums of its four children (p0 <= p1 <= p2 <= p3); pick the child whose
-// range holds k and make k relative to it.  Branch-free.
-__device__ __forceinline__ int descend4(const uint16_t* node, uint32_t& k)
-{
-    const uint64_t q = *(const uint64_t*)node;
-    const uint32_t p0 = (uint32_t)(q & 0xffff), p1 = (uint32_t)((q >> 16) & 0xffff);
-    const uint32_t p2 = (uint32_t)((q >> 32) & 0xffff);
-    const bool g0 = k >= p0, g1 = k >= p1, g2 = k >= p2;
-    const uint32_t sub = g2 ? p2 : (g1 ? p1 : (g0 ? p0 : 0u));
-    k -= sub;
-    return (int)g0 + (int)g1 + (int)g2;
-}
 
 // Block origin (x0, y0, z) of a level-4 key.
 __device__ __forceinline__ void block_origin(const Geom& g, uint32_t key, int& x0, int& y0, int& z)
@@ -808,10 +820,7 @@ __device__ __forceinline__ void emit_store(EmitWaveShared& ws, int p_lo, int p_h
         if (p_lo + q <= p_hi) {
             float* dst = ws.plane[(p_lo + q) % 3];
 #pragma unroll
-            for (int k = 0; k < kPlaneLoads; ++k) {
-                const int i = lane + 64 * k;
-                if (i < kBrickPlane) dst[i] = r[q][k];
-            }
+            for (int k = 0; k < kPlaneLoads; ++k) dst[lane + 64 * k] = r[q][k];
         }
     }
 }
@@ -882,7 +891,6 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
 
         const float* P0 = ws.plane[0];
         const float* P1 = ws.plane[1];
-        const float* P2 = ws.plane[NP - 1];
         const bool qvx0 = cx0 + qx < g.cx, qvx1 = cx0 + qx + 1 < g.cx;
         const bool qvy0 = cy0 + qy < g.cy, qvy1 = cy0 + qy + 1 < g.cy;
 
@@ -896,8 +904,7 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                 bits0 |= (P0[i] < T ? 1u : 0u) << (dx + 3 * dy);
                 bits1 |= (P1[i] < T ? 1u : 0u) << (dx + 3 * dy);
             }
-        uint32_t codes4 = 0, cnt4 = 0;
-        uint64_t l0q = 0;
+        uint32_t codes4 = 0, cnt4 = 0, cc[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int cxq = q & 1, cyq = q >> 1;
@@ -906,47 +913,62 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
             const uint32_t hi2 = ((bits1 >> sh0) & 3u) | (((bits1 >> (sh0 + 3)) & 3u) << 2);
             const uint32_t code = lo2 | (hi2 << 4);
             const bool valid = (cxq ? qvx1 : qvx0) && (cyq ? qvy1 : qvy0);
-            const uint32_t c = lut[code];
-            const uint32_t cv = valid ? c : 0u;
+            cc[q] = (uint32_t)lut[code] & (valid ? 0xffu : 0u);
             codes4 |= code << (8 * q);
-            cnt4 += cv;
-            l0q |= (uint64_t)cnt4 << (16 * q);   // level 0 node: inclusive prefixes
+            cnt4 += cc[q];
         }
-        // pyramid levels 1..3: every node stores its children's inclusive
-        // prefix sums, taken from one inclusive scan of the quad counts
-        uint32_t inc = cnt4;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t tt = __shfl_up(inc, o);
-            if (lane >= o) inc += tt;
-        }
-        const uint32_t exc = inc - cnt4;
-        const uint32_t base4 = __shfl(exc, lane & ~3);
-        const uint32_t base16 = __shfl(exc, lane & ~15);
-        *(uint64_t*)&ws.l0[4 * lane] = l0q;
+        // first vertex of every cell: an inclusive scan of the quad counts
+        // (cells in Morton order = the reference's child order,
+        // shadergen.cpp:448-495, so Morton order is its vertex order)
+        const uint32_t inc = wave_scan_u32<false>(cnt4);
+        const uint32_t e0 = inc - cnt4, e1 = e0 + cc[0], e2 = e1 + cc[1], e3 = e2 + cc[2];
         *(uint32_t*)&ws.code[4 * lane] = codes4;
-        ws.l1[lane] = (uint16_t)(inc - base4);
-        if ((lane & 3) == 3) ws.l2[lane >> 2] = (uint16_t)(inc - base16);
-        if ((lane & 15) == 15) ws.l3[lane >> 4] = (uint16_t)inc;
+        *(uint2*)&ws.cexc[4 * lane] = make_uint2(e0 | (e1 << 16), e2 | (e3 << 16));
+        ((uint32_t*)ws.vcell)[lane] = 0u;
         const uint32_t nv = min(__builtin_amdgcn_readlane(inc, 63), cnt);   // whole-block total
         const uint32_t vhi = min(vlo + (uint32_t)kEmitChunk, nv);
         wave_lds_sync();
+        // vertex -> cell for this chunk: each non-empty cell marks its first
+        // vertex (the cell holding vertex vlo marks position 0), then a max
+        // scan over the chunk carries every mark forward
+        {
+            // marks that do not apply go to this lane's spare byte
+            const uint32_t ex[4] = {e0, e1, e2, e3};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int pos = (int)ex[q] - (int)vlo;
+                const bool mark = cc[q] && pos < kEmitChunk && pos + (int)cc[q] > 0;
+                ws.vcell[mark ? max(pos, 0) : kEmitChunk + lane] = (uint8_t)(4 * lane + q);
+            }
+        }
+        wave_lds_sync();
+        {
+            const uint32_t w4 = ((const uint32_t*)ws.vcell)[lane];
+            const uint32_t m0 = w4 & 0xffu;
+            const uint32_t m1 = max(m0, (w4 >> 8) & 0xffu);
+            const uint32_t m2 = max(m1, (w4 >> 16) & 0xffu);
+            const uint32_t m3 = max(m2, w4 >> 24);
+            const uint32_t before = wave_shr1_u32(wave_scan_u32<true>(m3));
+            ((uint32_t*)ws.vcell)[lane] = max(before, m0) | (max(before, m1) << 8) | (max(before, m2) << 16) |
+                                          (max(before, m3) << 24);
+        }
+        wave_lds_sync();
 
         for (uint32_t v = vlo + lane; v < vhi; v += 64) {
-            uint32_t k = v;
-            int c = descend4(ws.l3, k);
-            c = 4 * c + descend4(ws.l2 + 4 * c, k);
-            c = 4 * c + descend4(ws.l1 + 4 * c, k);
-            c = 4 * c + descend4(ws.l0 + 4 * c, k);
-            const uint32_t slot = k;
-            const uint32_t e = (uint32_t)(cases[ws.code[c]] >> (4 + 4 * slot)) & 0xfu;
-            const int axis = (int)(e >> 2);
-            const int s0 = (int)(e & 1), s1 = (int)((e >> 1) & 1);
+            const int c = ws.vcell[v - vlo];
+            const uint32_t code = ws.code[c];
+            const uint32_t slot = v - ws.cexc[c];
+            const uint32_t e = (uint32_t)(cases[code] >> (4 + 4 * slot)) & 0xfu;
+            // edge e = 4 axis + s0 + 2 s1 -> corner offset (dx, dy, dz) of
+            // its first end and the axis, bits 0..4 of a 5-bit entry
+            constexpr uint64_t kEdgeOffsets =
+                (0x00ull << 0) | (0x02ull << 5) | (0x04ull << 10) | (0x06ull << 15) |     // axis x: (0,s0,s1)
+                (0x08ull << 20) | (0x09ull << 25) | (0x0cull << 30) | (0x0dull << 35) |   // axis y: (s0,0,s1)
+                (0x10ull << 40) | (0x11ull << 45) | (0x12ull << 50) | (0x13ull << 55);    // axis z: (s0,s1,0)
+            const uint32_t eo = (uint32_t)(kEdgeOffsets >> (5 * e)) & 31u;
+            const int axis = (int)(eo >> 3);
             const int ccx = (int)morton_compact((uint32_t)c), ccy = (int)morton_compact((uint32_t)c >> 1);
-            int ax = ccx, ay = ccy, az = 0;
-            if (axis == 0) { ay += s0; az = s1; }
-            else if (axis == 1) { ax += s0; az = s1; }
-            else { ax += s0; ay += s1; }
+            const int ax = ccx + (int)(eo & 1u), ay = ccy + (int)((eo >> 1) & 1u), az = (int)((eo >> 2) & 1u);
             const int bx_ = ax + (axis == 0), by_ = ay + (axis == 1), bz_ = az + (axis == 2);
             const float gx = (float)(cx0 + ax), gy = (float)(cy0 + ay), gz = (float)(cz + az);
             float nx, ny, nz, tt;
@@ -968,14 +990,13 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                 ny = T - (fa.y * u + fb.y * tt);
                 nz = T - (fa.z * u + fb.z * tt);
             } else {
-                const float* Pa = az ? P1 : P0;
-                const float* Pan = az ? P2 : P1;
-                const float* Pb = bz_ ? P1 : P0;
-                const float* Pbn = bz_ ? P2 : P1;
-                const int ia = ay * kBrick + ax, ib = by_ * kBrick + bx_;
-                const float va = Pa[ia], vb = Pb[ib];
-                const float nax = Pa[ia + 1], nay = Pa[ia + kBrick], naz = Pan[ia];
-                const float nbx_ = Pb[ib + 1], nby_ = Pb[ib + kBrick], nbz_ = Pbn[ib];
+                // sample (x, y, plane) at x + 18 y + 384 plane of the staged brick
+                const float* Pw = ws.plane[0];
+                const int ia = (int)__umul24((uint32_t)ay, (uint32_t)kBrick) + ax + az * kPlaneStride;
+                const int ib = (int)__umul24((uint32_t)by_, (uint32_t)kBrick) + bx_ + bz_ * kPlaneStride;
+                const float va = Pw[ia], vb = Pw[ib];
+                const float nax = Pw[ia + 1], nay = Pw[ia + kBrick], naz = Pw[ia + kPlaneStride];
+                const float nbx_ = Pw[ib + 1], nby_ = Pw[ib + kBrick], nbz_ = Pw[ib + kPlaneStride];
                 // va and vb lie on opposite sides of T, so va - vb != 0; the
                 // hardware reciprocal (1 ulp) keeps t within a few ulp
                 tt = (va - T) * __builtin_amdgcn_rcpf(va - vb);
@@ -1036,15 +1057,20 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
             }
             const uint32_t o = base + v;
             if (o < capacity) {
-                float2* dst = (float2*)(out + (size_t)o * 6);
-                dst[0] = make_float2(nx * g.scx, ny * g.scy);
-                dst[1] = make_float2(nz * g.scz, (gx + (axis == 0 ? tt : 0.0f)) * g.scx);
-                dst[2] = make_float2((gy + (axis == 1 ? tt : 0.0f)) * g.scy, (gz + (axis == 2 ? tt : 0.0f)) * g.scz);
+                // non-temporal: the output streams to memory instead of
+                // staying dirty in L2 (where its write-back would compete with
+                // the next build's classification)
+                typedef float v2f __attribute__((ext_vector_type(2)));
+                typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+                v2f* dst = (v2f*)(out + (size_t)o * 6);
+                __builtin_nontemporal_store((v2f){nx * g.scx, ny * g.scy}, dst);
+                __builtin_nontemporal_store((v2f){nz * g.scz, (gx + (axis == 0 ? tt : 0.0f)) * g.scx}, dst + 1);
+                __builtin_nontemporal_store(
+                    (v2f){(gy + (axis == 1 ? tt : 0.0f)) * g.scy, (gz + (axis == 2 ? tt : 0.0f)) * g.scz}, dst + 2);
                 if (out_ce) {
-                    CellEdge ce;
-                    ce.cell = (uint32_t)(cx0 + ccx) + (uint32_t)g.cx * ((uint32_t)(cy0 + ccy) + (uint32_t)g.cy * (uint32_t)cz);
-                    ce.edge = e;
-                    out_ce[o] = ce;
+                    const uint32_t cell = (uint32_t)(cx0 + ccx) +
+                                          (uint32_t)g.cx * ((uint32_t)(cy0 + ccy) + (uint32_t)g.cy * (uint32_t)cz);
+                    __builtin_nontemporal_store((v2u){cell, e}, (v2u*)(out_ce + o));
                 }
             }
         }
