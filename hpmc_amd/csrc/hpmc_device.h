@@ -771,6 +771,7 @@ struct EmitWaveShared {
     uint16_t cexc[256];            // per cell (Morton order): first vertex in the block
     uint8_t code[256];             // per cell: HPMC code
     uint8_t vcell[kEmitChunk + 64];  // per vertex of the item's chunk: its cell (+ a spare byte per lane)
+    float ostage[64 * 6];          // one batch of 64 output vertices, for contiguous stores
 };
 
 struct EmitShared {
@@ -954,7 +955,10 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
         }
         wave_lds_sync();
 
-        for (uint32_t v = vlo + lane; v < vhi; v += 64) {
+        // batches of 64 consecutive vertices, one per lane (lanes past the
+        // chunk's end repeat its last vertex and store nothing)
+        for (uint32_t v0 = vlo; v0 < vhi; v0 += 64) {
+            const uint32_t v = min(v0 + (uint32_t)lane, vhi - 1u);
             const int c = ws.vcell[v - vlo];
             const uint32_t code = ws.code[c];
             const uint32_t slot = v - ws.cexc[c];
@@ -1007,7 +1011,7 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                     // samples va, vb, na_q = f(a + e_q), nb_q = f(b + e_q)
                     // (na_axis is vb's sample) and to T
                     const uint32_t o = base + v;
-                    if (o >= capacity) continue;
+                    if (v0 + (uint32_t)lane >= vhi || o >= capacity) continue;
                     const float2* gi = (const float2*)(adj.gout + (size_t)o * 6);
                     const float2 g01 = gi[0], g23 = gi[1], g45 = gi[2];
                     const float wx = g01.x * g.scx, wy = g01.y * g.scy, wz = g23.x * g.scz;
@@ -1055,23 +1059,34 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                 ny = T - (nay * u + nby_ * tt);
                 nz = T - (naz * u + nbz_ * tt);
             }
-            const uint32_t o = base + v;
-            if (o < capacity) {
-                // non-temporal: the output streams to memory instead of
-                // staying dirty in L2 (where its write-back would compete with
-                // the next build's classification)
-                typedef float v2f __attribute__((ext_vector_type(2)));
-                typedef uint32_t v2u __attribute__((ext_vector_type(2)));
-                v2f* dst = (v2f*)(out + (size_t)o * 6);
-                __builtin_nontemporal_store((v2f){nx * g.scx, ny * g.scy}, dst);
-                __builtin_nontemporal_store((v2f){nz * g.scz, (gx + (axis == 0 ? tt : 0.0f)) * g.scx}, dst + 1);
-                __builtin_nontemporal_store(
-                    (v2f){(gy + (axis == 1 ? tt : 0.0f)) * g.scy, (gz + (axis == 2 ? tt : 0.0f)) * g.scz}, dst + 2);
-                if (out_ce) {
-                    const uint32_t cell = (uint32_t)(cx0 + ccx) +
-                                          (uint32_t)g.cx * ((uint32_t)(cy0 + ccy) + (uint32_t)g.cy * (uint32_t)cz);
-                    __builtin_nontemporal_store((v2u){cell, e}, (v2u*)(out_ce + o));
-                }
+            // The batch's 64 x 24 bytes go out as three contiguous 512-byte
+            // stores (lane L writes floats 2(64k + L), +1) after a transpose
+            // through LDS, non-temporal: the output streams to memory instead
+            // of staying dirty in L2, where its write-back would compete with
+            // the next build's classification.
+            typedef float v2f __attribute__((ext_vector_type(2)));
+            typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+            {
+                v2f* st = (v2f*)(ws.ostage + 6 * lane);
+                st[0] = (v2f){nx * g.scx, ny * g.scy};
+                st[1] = (v2f){nz * g.scz, (gx + (axis == 0 ? tt : 0.0f)) * g.scx};
+                st[2] = (v2f){(gy + (axis == 1 ? tt : 0.0f)) * g.scy, (gz + (axis == 2 ? tt : 0.0f)) * g.scz};
+            }
+            const uint32_t start = base + v0;   // first output vertex of the batch
+            const uint32_t nval = min(vhi - v0, 64u);
+            const uint32_t nout = capacity > start ? min(nval, capacity - start) : 0u;
+            wave_lds_sync();
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t pr = 64u * k + lane;   // float pair: vertex pr / 3
+                const v2f val = ((const v2f*)ws.ostage)[pr];
+                if (pr / 3u < nout) __builtin_nontemporal_store(val, (v2f*)(out + (size_t)start * 6) + pr);
+            }
+            wave_lds_sync();
+            if (out_ce && v0 + (uint32_t)lane < vhi && start + lane < capacity) {
+                const uint32_t cell = (uint32_t)(cx0 + ccx) +
+                                      (uint32_t)g.cx * ((uint32_t)(cy0 + ccy) + (uint32_t)g.cy * (uint32_t)cz);
+                __builtin_nontemporal_store((v2u){cell, e}, (v2u*)(out_ce + start + lane));
             }
         }
         if (!more) break;
