@@ -41,20 +41,13 @@ HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume4_gradient, HPMC_VOLUME4_FIELD, hpmc::kE
 HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume4_binary, HPMC_VOLUME4_FIELD, hpmc::kEmitBinary, false)
 HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume_batch, HPMC_VOLUME_FIELD, hpmc::kEmitContinuous, true)
 
-extern "C" __global__ __launch_bounds__(256) void hpmc_scan_reduce(const hpmc::VolDesc* vols, const uint32_t* tile_vol,
-                                                                   uint32_t n_l4t, const uint16_t* colcnt,
-                                                                   uint2* tile_sums)
-{
-    hpmc::scan_reduce_body(vols, tile_vol, n_l4t, colcnt, tile_sums);
-}
-
-extern "C" __global__ __launch_bounds__(256) void hpmc_scan_scatter(
+extern "C" __global__ __launch_bounds__(256) void hpmc_scan(
     const hpmc::VolDesc* vols, const uint32_t* tile_vol, uint32_t n_l4t, uint32_t n_colt, const uint16_t* colcnt,
-    const uint2* tile_sums, uint32_t* l4_off, uint2* items, hpmc::Totals* totals, hpmc::Totals* host_totals,
-    hpmc::VolCount* vol_counts, hpmc::VolCount* host_vol_counts)
+    uint2* tile_sums, hpmc::ScanState* state, uint32_t* l4_off, uint2* items, hpmc::Totals* totals,
+    hpmc::Totals* host_totals, hpmc::VolCount* vol_counts, hpmc::VolCount* host_vol_counts)
 {
-    hpmc::scan_scatter_body(vols, tile_vol, n_l4t, n_colt, colcnt, tile_sums, l4_off, items, totals, host_totals,
-                            vol_counts, host_vol_counts);
+    hpmc::scan_body(vols, tile_vol, n_l4t, n_colt, colcnt, tile_sums, state, l4_off, items, totals, host_totals,
+                    vol_counts, host_vol_counts);
 }
 
 extern "C" __global__ __launch_bounds__(256) void hpmc_level4_gather(const hpmc::VolDesc* vols, uint32_t vol,
@@ -575,8 +568,7 @@ bool launch_classify(HPMCHistoPyramid* h, float T)
 bool launch_scan(HPMCHistoPyramid* h)
 {
     stage_mark(h, kStageScan);
-    HIP_TRY(h, hpmc_host::plan_scan(h->plan, (const void*)hpmc_scan_reduce, (const void*)hpmc_scan_scatter,
-                                    h->stream));
+    HIP_TRY(h, hpmc_host::plan_scan(h->plan, (const void*)hpmc_scan, h->stream));
     HIP_TRY(h, hipGetLastError());
     stage_mark(h, kStageScan);
     return true;
@@ -1118,8 +1110,7 @@ int32_t batch_enqueue_build(HPMCBatch* b, float T)
 {
     BATCH_TRY(hpmc_host::plan_classify(b->plan, hpmc_host::Kernel{(const void*)hpmc_classify_volume, nullptr}, T,
                                        b->constants->d_cases, b->stream));
-    BATCH_TRY(hpmc_host::plan_scan(b->plan, (const void*)hpmc_scan_reduce, (const void*)hpmc_scan_scatter,
-                                   b->stream));
+    BATCH_TRY(hpmc_host::plan_scan(b->plan, (const void*)hpmc_scan, b->stream));
     return HPMC_OK;
 }
 
@@ -1247,9 +1238,10 @@ int32_t HPMCbatchGetCounts(struct HPMCBatch* b, uint32_t* counts, uint32_t* base
     BATCH_TRY(hipStreamSynchronize(b->stream));
     uint64_t sum = 0;
     for (int v = 0; v < b->plan.nvol; ++v) {
-        if (counts) counts[v] = b->plan.h_vcounts[v].count;
+        const uint32_t n = b->plan.h_vcounts[v].end - b->plan.h_vcounts[v].base;
+        if (counts) counts[v] = n;
         if (bases) bases[v] = b->plan.h_vcounts[v].base;
-        sum += b->plan.h_vcounts[v].count;
+        sum += n;
     }
     if (total) *total = sum;
     b->counts_known = true;

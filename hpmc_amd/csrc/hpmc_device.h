@@ -59,6 +59,17 @@ struct Geom {
 // column is written as one full line.
 constexpr int kColStride = 32;
 
+// Scan tiles: 4096 entries, 16 per thread of a 256-thread workgroup.
+constexpr int kScanTile = 4096;
+constexpr int kScanPer = kScanTile / 256;   // entries per thread
+
+// Emit work items hold at most kEmitChunk vertices of one block, so a busy
+// block is spread over several waves.
+constexpr int kEmitChunk = 256;
+static_assert(kEmitChunk == 256, "item_chunk_first assumes 256-vertex chunks");
+
+__device__ __forceinline__ uint32_t chunks_of(uint32_t cnt) { return (cnt + kEmitChunk - 1) / kEmitChunk; }
+
 
 struct Totals {
     uint32_t vertices;    // N
@@ -91,10 +102,10 @@ struct VolDesc {
     uint32_t lattice_base;    // first sample of the volume in a concatenated field (adjoint keys)
 };
 
-// Vertex base and count of one volume of a batch.
+// First and one-past-last vertex of one volume of a batch.
 struct VolCount {
     uint32_t base;
-    uint32_t count;
+    uint32_t end;
 };
 
 // Emit work item: x = level-4 key; y = chunk (bits 0..3, first vertex
@@ -386,7 +397,9 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
                                               uint16_t* __restrict__ colcnt, int zseg,
                                               const uint8_t* __restrict__ lut,
                                               uint32_t (*xch)[kClassifyMaxWaves],
-                                              uint16_t (*stage)[4][kColStride], int t)
+                                              uint16_t (*stage)[4][kColStride], int t,
+                                              uint2 (*red)[kColStride], uint2* __restrict__ tsum_l4,
+                                              uint2* __restrict__ tsum_col)
 {
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -474,19 +487,71 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
     // lane l writes entries 2(l%16), 2(l%16)+1 of block column l/16
     wave_lds_sync();
     const int bcol = (w.x0 >> kBlockLog2) + (lane >> 4);
+    const uint32_t wi = ((uint32_t)pz * g.nby + py) * g.nbx + bcol;
+    uint32_t vsum = 0, isum = 0;
     if (bcol < g.nbx) {
-        const uint32_t pair = (uint32_t)mycnt[lane >> 4][(lane & 15) * 2] |
-                              ((uint32_t)mycnt[lane >> 4][(lane & 15) * 2 + 1] << 16);
-        uint32_t* col32 = (uint32_t*)(colcnt + (((size_t)pz * g.nby + py) * g.nbx + bcol) * kColStride);
-        col32[lane & 15] = pair;
+        const uint32_t lo = mycnt[lane >> 4][(lane & 15) * 2], hi = mycnt[lane >> 4][(lane & 15) * 2 + 1];
+        uint32_t* col32 = (uint32_t*)(colcnt + (size_t)wi * kColStride);
+        col32[lane & 15] = lo | (hi << 16);
+        vsum = lo + hi;
+        isum = chunks_of(lo) + chunks_of(hi);
+    }
+
+    // Scan tile sums, added after the last load so that the atomics never
+    // hold up the stream.  Column tiles: each block column's vertices and
+    // emit items.
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+        vsum += __shfl_xor(vsum, o, 16);
+        isum += __shfl_xor(isum, o, 16);
+    }
+    if ((lane & 15) == 0 && vsum) {
+        const uint32_t t = wi * (uint32_t)kColStride / (uint32_t)kScanTile;
+        atomicAdd(&tsum_col[t].x, vsum);
+        atomicAdd(&tsum_col[t].y, isum);
+    }
+    // Level-4 tiles: per slice, the vertices and non-empty blocks of the
+    // wave's four blocks (which share one Morton tile).  When a whole slice
+    // lies in one Morton tile (slice tiles of at most 64 x 64 blocks) the
+    // workgroup's waves are summed first, one atomic pair per slice.
+    const int ns = z1 - z0;
+    uint32_t sv = 0, sb = 0;
+    if (lane < ns) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t cj = mycnt[j][lane];
+            sv += cj;
+            sb += cj != 0;
+        }
+    }
+    const bool whole = g.tsx2 <= 64 * kBlock && g.tsy2 <= 64 * kBlock;
+    if (!whole) {
+        if (lane < ns && sv) {
+            const uint32_t t = block_key(g, w.x0, w.y0, z0 + lane) / (uint32_t)kScanTile;
+            atomicAdd(&tsum_l4[t].x, sv);
+            atomicAdd(&tsum_l4[t].y, sb);
+        }
+        return;
+    }
+    if (lane < ns) red[wv][lane] = make_uint2(sv, sb);
+    __syncthreads();
+    if (wv == 0 && lane < ns) {
+        uint2 r = make_uint2(0, 0);
+        for (int i = 0; i < nw; ++i) {
+            r.x += red[i][lane].x;
+            r.y += red[i][lane].y;
+        }
+        if (r.x) {
+            const uint32_t t = block_key(g, 0, w.y0, z0 + lane) / (uint32_t)kScanTile;
+            atomicAdd(&tsum_l4[t].x, r.x);
+            atomicAdd(&tsum_l4[t].y, r.y);
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// scan, part 1: per 4096-entry tile of the level-4 array, the sum of counts
-// and of non-empty blocks.
-constexpr int kScanTile = 4096;
-constexpr int kScanPer = kScanTile / 256;   // entries per thread
+// ---------------------------------------------------------------------------
+// scan helpers
 
 __device__ __forceinline__ uint2 block_reduce_u2(uint2 v, uint2* sh)
 {
@@ -528,12 +593,6 @@ __device__ __forceinline__ void scan_load16(const uint32_t* __restrict__ level4,
     }
 }
 
-// Emit work items hold at most kEmitChunk vertices of one block, so a busy
-// block is spread over several waves.
-constexpr int kEmitChunk = 256;
-static_assert(kEmitChunk == 256, "item_chunk_first assumes 256-vertex chunks");
-
-__device__ __forceinline__ uint32_t chunks_of(uint32_t cnt) { return (cnt + kEmitChunk - 1) / kEmitChunk; }
 
 // The 16 column-major block counts of thread t of a tile (u16, two 16-byte
 // loads); entries past n read as zero.
@@ -580,53 +639,46 @@ __device__ __forceinline__ void scan_load_tile(const VolDesc& d, const uint16_t*
     else scan_load16_u16(colcnt + d.colcnt_off, d.n_cc, base, c);
 }
 
-__device__ __forceinline__ void scan_reduce_body(const VolDesc* __restrict__ vols,
-                                                 const uint32_t* __restrict__ tile_vol, uint32_t n_l4t,
-                                                 const uint16_t* __restrict__ colcnt, uint2* __restrict__ tile_sums)
-{
-    __shared__ uint2 sh[16];
-    const uint32_t tile = blockIdx.x;
-    const bool is_col = tile >= n_l4t;
-    const VolDesc d = vols[__builtin_amdgcn_readfirstlane(tile_vol[tile])];
-    const uint32_t local = is_col ? tile - n_l4t - d.col_tile0 : tile - d.l4_tile0;
-    uint32_t c[kScanPer];
-    scan_load_tile(d, colcnt, is_col, local, c);
-    uint2 s = make_uint2(0, 0);
-#pragma unroll
-    for (int k = 0; k < kScanPer; ++k) {
-        s.x += c[k];
-        s.y += is_col ? chunks_of(c[k]) : (uint32_t)(c[k] != 0);
-    }
-    s = block_reduce_u2(s, sh);
-    if (threadIdx.x == 0) tile_sums[tile] = s;
-}
+// One-pass scan.  The tile sums were accumulated by the classification
+// kernel, so every tile reads the sums of the tiles before it in its
+// sequence (a few hundred), scans its own 4096 entries and scatters.
+// Level-4 tiles write, for each non-empty block, the global offset of its
+// first vertex: the concatenation of the volumes' vertex lists, each in
+// Morton key order (the reference's key order); a volume's first level-4
+// tile writes its vertex base and end.  Column tiles write the emit work
+// items -- one per kEmitChunk vertices of a block -- in (volume, z segment,
+// y, x, z) order, so that consecutive items are consecutive slices of one
+// block column.  Tile 0 writes the totals, to device memory and to the
+// host-visible copy.  The last workgroup to finish clears the tile sums for
+// the next build.
+struct ScanState {
+    uint32_t done;            // workgroups finished
+    uint32_t pad[3];
+};
 
-// Every tile re-reads the sums of the tiles before it in its sequence (a
-// few hundred) and scans its own 4096 entries.  Level-4 tiles write, for
-// each non-empty block, the global offset of its first vertex: the
-// concatenation of the volumes' vertex lists, each in Morton key order (the
-// reference's key order).  The first level-4 tile of a volume also writes
-// the volume's vertex base and count; tile 0 the totals, to device memory
-// and to the host-visible copies.  Column tiles write the emit work items --
-// one per kEmitChunk vertices of a block -- in (volume, z segment, y, x, z)
-// order, so that consecutive items are consecutive slices of one block column.
-__device__ __forceinline__ void scan_scatter_body(const VolDesc* __restrict__ vols,
-                                                  const uint32_t* __restrict__ tile_vol, uint32_t n_l4t,
-                                                  uint32_t n_colt, const uint16_t* __restrict__ colcnt,
-                                                  const uint2* __restrict__ tile_sums,
-                                                  uint32_t* __restrict__ l4_off, uint2* __restrict__ items,
-                                                  Totals* __restrict__ totals, Totals* __restrict__ host_totals,
-                                                  VolCount* __restrict__ vol_counts,
-                                                  VolCount* __restrict__ host_vol_counts)
+__device__ __forceinline__ void scan_body(
+    const VolDesc* __restrict__ vols, const uint32_t* __restrict__ tile_vol, uint32_t n_l4t, uint32_t n_colt,
+    const uint16_t* __restrict__ colcnt, uint2* __restrict__ tile_sums, ScanState* __restrict__ state,
+    uint32_t* __restrict__ l4_off, uint2* __restrict__ items, Totals* __restrict__ totals,
+    Totals* __restrict__ host_totals, VolCount* __restrict__ vol_counts, VolCount* __restrict__ host_vol_counts)
 {
     __shared__ uint2 sh[16];
     __shared__ uint2 wsum[16];
+    __shared__ uint32_t s_last;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t ntiles = n_l4t + n_colt;
     const uint32_t tile = blockIdx.x;
     const bool is_col = tile >= n_l4t;
+    const uint32_t t0 = is_col ? n_l4t : 0u;
+    // preceding tile sums, loaded alongside the tile's own entries
+    uint2 pre = make_uint2(0, 0);
+    for (uint32_t i = t0 + threadIdx.x; i < tile; i += blockDim.x) {
+        const uint2 v = tile_sums[i];
+        pre.x += v.x;
+        pre.y += v.y;
+    }
     const uint32_t vol = __builtin_amdgcn_readfirstlane(tile_vol[tile]);
     const VolDesc d = vols[vol];
-    const uint32_t t0 = is_col ? n_l4t : 0u;
     const uint32_t local = is_col ? tile - n_l4t - d.col_tile0 : tile - d.l4_tile0;
     uint32_t c[kScanPer];
     scan_load_tile(d, colcnt, is_col, local, c);
@@ -636,22 +688,16 @@ __device__ __forceinline__ void scan_scatter_body(const VolDesc* __restrict__ vo
         if (!is_col) mine.x += c[k];
         else mine.y += chunks_of(c[k]);
     }
-    // sums of the preceding tiles of this sequence
-    uint2 pre = make_uint2(0, 0);
-    for (uint32_t i = t0 + threadIdx.x; i < tile; i += blockDim.x) {
-        pre.x += tile_sums[i].x;
-        pre.y += tile_sums[i].y;
-    }
     pre = block_reduce_u2(pre, sh);
     if (!is_col && local == 0) {
-        // this volume's vertex base and count
+        // this volume's vertex base and end
         uint32_t cnt = 0;
         for (uint32_t i = threadIdx.x; i < d.n_l4tiles; i += blockDim.x) cnt += tile_sums[d.l4_tile0 + i].x;
         cnt = block_reduce_u2(make_uint2(cnt, 0), sh).x;
         if (threadIdx.x == 0) {
             VolCount vc;
             vc.base = pre.x;
-            vc.count = cnt;
+            vc.end = pre.x + cnt;
             vol_counts[vol] = vc;
             if (host_vol_counts) host_vol_counts[vol] = vc;
         }
@@ -714,6 +760,14 @@ __device__ __forceinline__ void scan_scatter_body(const VolDesc* __restrict__ vo
             const uint32_t yv = (c[k] << 4) | (vol << 16);
             for (uint32_t ch = 0; ch * kEmitChunk < c[k]; ++ch) items[off.y++] = make_uint2(key, yv | ch);
         }
+    }
+    // the last workgroup to finish clears the tile sums for the next build
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(&state->done, 1u) == ntiles - 1;
+    __syncthreads();
+    if (s_last) {
+        for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) tile_sums[i] = make_uint2(0, 0);
+        if (threadIdx.x == 0) state->done = 0u;
     }
 }
 
@@ -1183,11 +1237,12 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
 #define HPMC_DEFINE_BUILD_KERNELS(SUFFIX, MAKE_FIELD)                                          \
     extern "C" __global__ __launch_bounds__(1024) void hpmc_classify_##SUFFIX(                 \
         const hpmc::VolDesc* vols, const uint32_t* cls_vol, uint32_t n_wg, float T,            \
-        uint16_t* colcnt, const uint64_t* cases)                                              \
+        uint16_t* colcnt, const uint64_t* cases, uint2* tile_sums, uint32_t n_l4t)             \
     {                                                                                         \
         __shared__ uint8_t lut[256];                                                          \
         __shared__ uint32_t xch[2][hpmc::kClassifyMaxWaves];                                   \
         __shared__ uint16_t stage[hpmc::kClassifyMaxWaves][4][hpmc::kColStride];               \
+        __shared__ uint2 red[hpmc::kClassifyMaxWaves][hpmc::kColStride];                       \
         for (int i = threadIdx.x; i < 256; i += blockDim.x) lut[i] = (uint8_t)(cases[i] & 0xfu); \
         __syncthreads();                                                                      \
         /* grid padded to a multiple of 8: XCD x takes logical run x */                       \
@@ -1195,7 +1250,8 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
         if (t >= n_wg) return;                                                                \
         const hpmc::VolDesc dd = vols[__builtin_amdgcn_readfirstlane(cls_vol[t])];            \
         hpmc::classify_body(MAKE_FIELD, dd.g, T, colcnt + dd.colcnt_off, dd.g.zseg, lut, xch, \
-                            stage, (int)(t - dd.cls_wg0));                                    \
+                            stage, (int)(t - dd.cls_wg0), red, tile_sums + dd.l4_tile0,       \
+                            tile_sums + n_l4t + dd.col_tile0);                                \
     }                                                                                         \
     extern "C" __global__ __launch_bounds__(256) void hpmc_codes_##SUFFIX(                     \
         const hpmc::VolDesc* vols, uint32_t vol, float T, uint8_t* codes)                     \

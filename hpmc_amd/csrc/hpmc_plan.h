@@ -10,7 +10,8 @@
 //              slices per block column (whole 64-byte lines)
 //   l4_off     u32 per (volume, level-4 key): global offset of the block's
 //              first vertex in the concatenated output
-//   tile_sums  uint2 per scan tile
+//   tile_sums  uint2 per scan tile, accumulated by classify (atomics) and
+//              cleared by the scan's last workgroup; scan_state its counter
 //   items      uint2 per emit work item (<= 15 per non-empty block slice)
 //   totals / vol_counts: grand totals and per-volume (base, count), also
 //              written straight into pinned host memory
@@ -158,6 +159,7 @@ struct Plan {
     uint16_t* colcnt = nullptr;
     uint32_t* l4_off = nullptr;
     uint2* tile_sums = nullptr;
+    hpmc::ScanState* scan_state = nullptr;
     uint2* items = nullptr;
     hpmc::Totals* d_totals = nullptr;
     hpmc::Totals* h_totals = nullptr;     // pinned
@@ -173,6 +175,7 @@ inline void plan_free(Plan& p)
     if (p.colcnt) (void)hipFree(p.colcnt);
     if (p.l4_off) (void)hipFree(p.l4_off);
     if (p.tile_sums) (void)hipFree(p.tile_sums);
+    if (p.scan_state) (void)hipFree(p.scan_state);
     if (p.items) (void)hipFree(p.items);
     if (p.d_totals) (void)hipFree(p.d_totals);
     if (p.h_totals) (void)hipHostFree(p.h_totals);
@@ -306,6 +309,9 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
     PLAN_TRY(hipMemset(p.colcnt, 0, p.n_cc * sizeof(uint16_t)));
     PLAN_TRY(hipMalloc(&p.l4_off, p.n_l4 * sizeof(uint32_t)));
     PLAN_TRY(hipMalloc(&p.tile_sums, (size_t)(p.n_l4t + p.n_colt) * sizeof(uint2)));
+    PLAN_TRY(hipMemset(p.tile_sums, 0, (size_t)(p.n_l4t + p.n_colt) * sizeof(uint2)));
+    PLAN_TRY(hipMalloc(&p.scan_state, sizeof(hpmc::ScanState)));
+    PLAN_TRY(hipMemset(p.scan_state, 0, sizeof(hpmc::ScanState)));
     PLAN_TRY(hipMalloc(&p.items, std::max<uint64_t>(p.n_items_max, 1) * sizeof(uint2)));
     PLAN_TRY(hipMalloc(&p.d_totals, sizeof(hpmc::Totals)));
     PLAN_TRY(hipMemset(p.d_totals, 0, sizeof(hpmc::Totals)));
@@ -347,13 +353,15 @@ inline hipError_t plan_classify(const Plan& p, const Kernel& k, float T, const u
     float Ta = T;
     uint16_t* cc = p.colcnt;
     const uint64_t* cs = cases;
-    void* args[] = {&vols, &cv, &n, &Ta, &cc, &cs};
+    uint2* ts = p.tile_sums;
+    uint32_t nl = p.n_l4t;
+    void* args[] = {&vols, &cv, &n, &Ta, &cc, &cs, &ts, &nl};
     // the XCD-aware workgroup mapping needs the grid padded to a multiple of 8
     const unsigned grid = (unsigned)(((uint64_t)p.n_wg + 7) / 8 * 8);
     return launch(k, grid, 64u * (unsigned)p.cls_waves, args, stream);
 }
 
-inline hipError_t plan_scan(const Plan& p, const void* k_reduce, const void* k_scatter, hipStream_t stream)
+inline hipError_t plan_scan(const Plan& p, const void* k_scan, hipStream_t stream)
 {
     const unsigned grid = p.n_l4t + p.n_colt;
     const hpmc::VolDesc* vols = p.d_vols;
@@ -361,18 +369,15 @@ inline hipError_t plan_scan(const Plan& p, const void* k_reduce, const void* k_s
     uint32_t nl = p.n_l4t, nc = p.n_colt;
     const uint16_t* cc = p.colcnt;
     uint2* ts = p.tile_sums;
-    void* a1[] = {&vols, &tv, &nl, &cc, &ts};
-    hipError_t e = hipLaunchKernel(k_reduce, dim3(grid), dim3(256), a1, 0, stream);
-    if (e != hipSuccess) return e;
-    const uint2* tsc = p.tile_sums;
+    hpmc::ScanState* st = p.scan_state;
     uint32_t* lo = p.l4_off;
     uint2* it = p.items;
     hpmc::Totals* dt = p.d_totals;
     hpmc::Totals* ht = p.h_totals;
     hpmc::VolCount* dv = p.d_vcounts;
     hpmc::VolCount* hv = p.h_vcounts;
-    void* a2[] = {&vols, &tv, &nl, &nc, &cc, &tsc, &lo, &it, &dt, &ht, &dv, &hv};
-    return hipLaunchKernel(k_scatter, dim3(grid), dim3(256), a2, 0, stream);
+    void* args[] = {&vols, &tv, &nl, &nc, &cc, &ts, &st, &lo, &it, &dt, &ht, &dv, &hv};
+    return hipLaunchKernel(k_scan, dim3(grid), dim3(256), args, 0, stream);
 }
 
 inline hipError_t plan_emit(const Plan& p, const Kernel& k, float T, float* out, hpmc::CellEdge* out_ce,
