@@ -847,7 +847,9 @@ __device__ __forceinline__ void block_origin(const Geom& g, uint32_t key, int& x
 
 // Loads planes p_lo..p_hi (1 to 3) of the 18x18 window at (x0, y0) into r.
 template <class F>
-__device__ __forceinline__ void emit_fetch(const F& field, int x0, int y0, int p_lo, int p_hi, int lane,
+// Loads planes zfirst .. zfirst + n - 1 (n = 0..3, wave-uniform) of the
+// 18x18 window at (x0, y0) into r[0 .. n-1].
+__device__ __forceinline__ void emit_fetch(const F& field, int x0, int y0, int zfirst, int n, int lane,
                                            float (&r)[3][kPlaneLoads])
 {
     uint32_t idx[kPlaneLoads];
@@ -859,21 +861,23 @@ __device__ __forceinline__ void emit_fetch(const F& field, int x0, int y0, int p
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        if (p_lo + q <= p_hi) {
-            const typename F::Plane pl = field.plane(p_lo + q);
+        if (q < n) {
+            const typename F::Plane pl = field.plane(zfirst + q);
 #pragma unroll
             for (int k = 0; k < kPlaneLoads; ++k) r[q][k] = field.fetch(pl, idx[k]);
         }
     }
 }
 
-__device__ __forceinline__ void emit_store(EmitWaveShared& ws, int p_lo, int p_hi, int lane,
+// Stores planes zfirst .. zfirst + n - 1 from r into their ring slots
+// (plane z in slot z mod 3).
+__device__ __forceinline__ void emit_store(EmitWaveShared& ws, int zfirst, int n, int lane,
                                            const float (&r)[3][kPlaneLoads])
 {
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        if (p_lo + q <= p_hi) {
-            float* dst = ws.plane[(p_lo + q) % 3];
+        if (q < n) {
+            float* dst = ws.plane[(zfirst + q) % 3];
 #pragma unroll
             for (int k = 0; k < kPlaneLoads; ++k) dst[lane + 64 * k] = r[q][k];
         }
@@ -896,12 +900,17 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     const int wv = threadIdx.x >> 6;
     const uint32_t nitems = __builtin_amdgcn_readfirstlane(totals->items);
     const uint32_t nwg = gridDim.x;                 // a multiple of 8
-    const uint32_t stride = nwg * kEmitWaves;
     // workgroups are dealt to the 8 XCDs round robin; give each XCD a
-    // contiguous run of logical workgroup indices
+    // contiguous run of logical workgroup indices, and each wave a
+    // contiguous run of items: consecutive items are mostly the next slice
+    // of the same block column (or the next chunk of the same block), whose
+    // planes are already staged
     const uint32_t t = (blockIdx.x & 7u) * (nwg >> 3) + (blockIdx.x >> 3);
-    uint32_t it = __builtin_amdgcn_readfirstlane(t * kEmitWaves + wv);
-    if (it >= nitems) return;
+    const uint64_t nwaves = (uint64_t)nwg * kEmitWaves;
+    const uint64_t wlog = (uint64_t)t * kEmitWaves + wv;
+    uint32_t it = __builtin_amdgcn_readfirstlane((uint32_t)(wlog * nitems / nwaves));
+    const uint32_t iend = __builtin_amdgcn_readfirstlane((uint32_t)((wlog + 1) * nitems / nwaves));
+    if (it >= iend) return;
 
     EmitWaveShared& ws = sh.w[wv];
     const uint64_t* cases = sh.cases;
@@ -918,21 +927,23 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     float r[3][kPlaneLoads];
     int x0, y0, z;
     block_origin(dn.g, cur.x, x0, y0, z);
-    emit_fetch(mk(dn), x0, y0, z, z + NP - 1, lane, r);
-    uint2 nxt = items[min(it + stride, nitems - 1)];
+    int nnew = NP;                      // planes the item needs that are not staged yet
+    emit_fetch(mk(dn), x0, y0, z, NP, lane, r);
+    uint2 nxt = items[min(it + 1, iend - 1)];
 
     for (;;) {
         const uint32_t vlo = item_chunk_first(cur.y), cnt = item_block_count(cur.y);
         const uint32_t base = __builtin_amdgcn_readfirstlane(obase);
         const int cz = z, cx0 = x0, cy0 = y0;
+        const uint32_t cvol = BATCH ? item_volume(cur.y) : 0u;
         const VolDesc dc = dn;          // this item's volume
         const Geom& g = dc.g;
         wave_lds_sync();
-        emit_store(ws, 0, NP - 1, lane, r);
+        emit_store(ws, cz + NP - nnew, nnew, lane, r);
         wave_lds_sync();
 
-        // next item: entry, output offset and planes in flight meanwhile
-        const bool more = it + stride < nitems;
+        // next item: entry, output offset and missing planes in flight meanwhile
+        const bool more = it + 1 < iend;
         uint2 after = nxt;
         if (more) {
             nxt.x = __builtin_amdgcn_readfirstlane(nxt.x);
@@ -940,12 +951,16 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
             if constexpr (BATCH) dn = vols[item_volume(nxt.y)];
             obase = l4_off[dn.l4_base + nxt.x];
             block_origin(dn.g, nxt.x, x0, y0, z);
-            emit_fetch(mk(dn), x0, y0, z, z + NP - 1, lane, r);
-            after = items[min(it + 2 * stride, nitems - 1)];
+            const bool column = x0 == cx0 && y0 == cy0 && (!BATCH || item_volume(nxt.y) == cvol);
+            nnew = (column && z == cz) ? 0 : ((column && z == cz + 1) ? 1 : NP);
+            emit_fetch(mk(dn), x0, y0, z + NP - nnew, nnew, lane, r);
+            after = items[min(it + 2, iend - 1)];
         }
 
-        const float* P0 = ws.plane[0];
-        const float* P1 = ws.plane[1];
+        // plane ring: plane cz + a sits in slot (cz + a) mod 3
+        const int s0 = cz % 3, s1 = (cz + 1) % 3, s2 = (cz + 2) % 3;
+        const float* P0 = ws.plane[s0];
+        const float* P1 = ws.plane[s1];
         const bool qvx0 = cx0 + qx < g.cx, qvx1 = cx0 + qx + 1 < g.cx;
         const bool qvy0 = cy0 + qy < g.cy, qvy1 = cy0 + qy + 1 < g.cy;
 
@@ -1048,13 +1063,17 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                 ny = T - (fa.y * u + fb.y * tt);
                 nz = T - (fa.z * u + fb.z * tt);
             } else {
-                // sample (x, y, plane) at x + 18 y + 384 plane of the staged brick
+                // sample (x, y, plane) at x + 18 y + 384 slot of the staged
+                // brick; a and b lie in planes 0 or 1, their +z neighbours in
+                // planes 1 or 2 (an edge along z starts in plane 0)
                 const float* Pw = ws.plane[0];
-                const int ia = (int)__umul24((uint32_t)ay, (uint32_t)kBrick) + ax + az * kPlaneStride;
-                const int ib = (int)__umul24((uint32_t)by_, (uint32_t)kBrick) + bx_ + bz_ * kPlaneStride;
-                const float va = Pw[ia], vb = Pw[ib];
-                const float nax = Pw[ia + 1], nay = Pw[ia + kBrick], naz = Pw[ia + kPlaneStride];
-                const float nbx_ = Pw[ib + 1], nby_ = Pw[ib + kBrick], nbz_ = Pw[ib + kPlaneStride];
+                const int o0 = s0 * kPlaneStride, o1 = s1 * kPlaneStride, o2 = s2 * kPlaneStride;
+                const int ia = (int)__umul24((uint32_t)ay, (uint32_t)kBrick) + ax;
+                const int ib = (int)__umul24((uint32_t)by_, (uint32_t)kBrick) + bx_;
+                const int pa = az ? o1 : o0, pa1 = az ? o2 : o1, pb = bz_ ? o1 : o0, pb1 = bz_ ? o2 : o1;
+                const float va = Pw[ia + pa], vb = Pw[ib + pb];
+                const float nax = Pw[ia + pa + 1], nay = Pw[ia + pa + kBrick], naz = Pw[ia + pa1];
+                const float nbx_ = Pw[ib + pb + 1], nby_ = Pw[ib + pb + kBrick], nbz_ = Pw[ib + pb1];
                 // va and vb lie on opposite sides of T, so va - vb != 0; the
                 // hardware reciprocal (1 ulp) keeps t within a few ulp
                 tt = (va - T) * __builtin_amdgcn_rcpf(va - vb);
@@ -1144,7 +1163,7 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
             }
         }
         if (!more) break;
-        it += stride;
+        ++it;
         cur = nxt;
         nxt = after;
     }
