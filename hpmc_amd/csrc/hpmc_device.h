@@ -183,6 +183,25 @@ __device__ __forceinline__ uint32_t wave_scan_u32(uint32_t x)
     return x;
 }
 
+// A load from a wave-uniform address through the scalar data cache (read
+// only; the data was written by an earlier kernel).  Uniform loads must not
+// go through the vector memory path: a vector load's result has to be made
+// uniform with v_readfirstlane, and the vmcnt wait that needs would also
+// wait for every vector load issued before it (the prefetches).
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* p)
+{
+    static_assert(sizeof(T) % 4 == 0, "loaded as 32-bit words");
+    typedef const __attribute__((address_space(4))) uint32_t* cptr;
+    const cptr q = (cptr)p;
+    uint32_t w[sizeof(T) / 4];
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 4; ++i) w[i] = q[i];
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+}
+
 // The value of lane - 1 (0 in lane 0): DPP wave_shr:1.
 __device__ __forceinline__ uint32_t wave_shr1_u32(uint32_t x)
 {
@@ -898,7 +917,7 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     constexpr int NP = emit_planes<MODE>();
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const uint32_t nitems = __builtin_amdgcn_readfirstlane(totals->items);
+    const uint32_t nitems = load_uniform(&totals->items);
     const uint32_t nwg = gridDim.x;                 // a multiple of 8
     // workgroups are dealt to the 8 XCDs round robin; give each XCD a
     // contiguous run of logical workgroup indices, and each wave a
@@ -919,21 +938,19 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     const int qx = (int)morton_compact((uint32_t)lane) * 2;
     const int qy = (int)morton_compact((uint32_t)lane >> 1) * 2;
 
-    uint2 cur = items[it];
-    cur.x = __builtin_amdgcn_readfirstlane(cur.x);
-    cur.y = __builtin_amdgcn_readfirstlane(cur.y);
-    VolDesc dn = vols[BATCH ? item_volume(cur.y) : 0u];
-    uint32_t obase = l4_off[dn.l4_base + cur.x];
+    uint2 cur = load_uniform(items + it);
+    VolDesc dn = load_uniform(vols + (BATCH ? item_volume(cur.y) : 0u));
+    uint32_t obase = load_uniform(l4_off + dn.l4_base + cur.x);
     float r[3][kPlaneLoads];
     int x0, y0, z;
     block_origin(dn.g, cur.x, x0, y0, z);
     int nnew = NP;                      // planes the item needs that are not staged yet
     emit_fetch(mk(dn), x0, y0, z, NP, lane, r);
-    uint2 nxt = items[min(it + 1, iend - 1)];
+    uint2 nxt = load_uniform(items + min(it + 1, iend - 1));
 
     for (;;) {
         const uint32_t vlo = item_chunk_first(cur.y), cnt = item_block_count(cur.y);
-        const uint32_t base = __builtin_amdgcn_readfirstlane(obase);
+        const uint32_t base = obase;
         const int cz = z, cx0 = x0, cy0 = y0;
         const uint32_t cvol = BATCH ? item_volume(cur.y) : 0u;
         const VolDesc dc = dn;          // this item's volume
@@ -946,15 +963,13 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
         const bool more = it + 1 < iend;
         uint2 after = nxt;
         if (more) {
-            nxt.x = __builtin_amdgcn_readfirstlane(nxt.x);
-            nxt.y = __builtin_amdgcn_readfirstlane(nxt.y);
-            if constexpr (BATCH) dn = vols[item_volume(nxt.y)];
-            obase = l4_off[dn.l4_base + nxt.x];
+            if constexpr (BATCH) dn = load_uniform(vols + item_volume(nxt.y));
+            obase = load_uniform(l4_off + dn.l4_base + nxt.x);
             block_origin(dn.g, nxt.x, x0, y0, z);
             const bool column = x0 == cx0 && y0 == cy0 && (!BATCH || item_volume(nxt.y) == cvol);
             nnew = (column && z == cz) ? 0 : ((column && z == cz + 1) ? 1 : NP);
             emit_fetch(mk(dn), x0, y0, z + NP - nnew, nnew, lane, r);
-            after = items[min(it + 2, iend - 1)];
+            after = load_uniform(items + min(it + 2, iend - 1));
         }
 
         // plane ring: plane cz + a sits in slot (cz + a) mod 3
