@@ -59,6 +59,8 @@ SIGNATURES = [
     ("HPMCgetLastError", _i32, [_vp]),
     ("HPMCgetLastErrorString", ctypes.c_char_p, [_vp]),
     ("HPMCsetStream", _i32, [_vp, _vp]),
+    ("HPMCsetTuning", _i32, [_vp, _i32, _i32, _i32]),
+    ("HPMCgetTuning", _i32, [_vp, ctypes.POINTER(_i32)]),
     ("HPMCsetFieldUniform", _i32, [_vp, ctypes.c_char_p, _vp, ctypes.c_size_t]),
     ("HPMCsetTraversalHandleBuffer", _i32, [_vp, _vp, _u64]),
     ("HPMCsetTraversalHandleCellEdgeBuffer", _i32, [_vp, _vp, _u64]),
@@ -279,6 +281,16 @@ class IsoSurface:
         _check(lib().HPMCadjoint(self._traversal(), ctypes.c_void_p(gv.data_ptr()) if gv is not None else None,
                                  ctypes.c_void_p(gf.data_ptr()), ctypes.c_void_p(gt.data_ptr())), self.h)
         return gf, gt
+
+    def set_tuning(self, classify_waves=0, zseg=0, emit_workgroups=0):
+        """Launch shapes (0 = automatic); results do not depend on them."""
+        _check(lib().HPMCsetTuning(self.h, classify_waves, zseg, emit_workgroups), self.h)
+
+    def tuning(self):
+        """(classify_waves, zseg, emit_workgroups) in use."""
+        out = (ctypes.c_int32 * 3)()
+        _check(lib().HPMCgetTuning(self.h, out), self.h)
+        return tuple(out)
 
     def set_profiling(self, enable=True):
         _check(lib().HPMCsetProfiling(self.h, int(bool(enable))), self.h)

@@ -286,6 +286,7 @@ struct HPMCHistoPyramid {
     uint32_t count = 0;
     float threshold = 0.0f;
     hipStream_t stream = nullptr;
+    hpmc_host::Tuning tuning;        // launch shapes (0 = automatic)
     // hipRTC module for snippets
     hipModule_t module = nullptr;
     uint32_t builder_id = 0;
@@ -529,7 +530,7 @@ bool setup(HPMCHistoPyramid* h)
     }
     spec.field = h->mode == kFieldVolume ? (const void*)h->volume : nullptr;
     std::string err;
-    const int32_t rc = hpmc_host::plan_create(h->plan, {spec}, hpmc_host::Tuning(),
+    const int32_t rc = hpmc_host::plan_create(h->plan, {spec}, h->tuning,
                                               (const void*)hpmc_classify_volume, device_shape(), &err);
     if (rc != HPMC_OK) {
         fail(h, rc, "%s", err.c_str());
@@ -873,6 +874,31 @@ int32_t HPMCsetStream(struct HPMCHistoPyramid* h, void* stream)
 {
     if (!h) return fail(nullptr, HPMC_ERR_NULL, "h == NULL");
     h->stream = (hipStream_t)stream;
+    ok(h);
+    return HPMC_OK;
+}
+
+int32_t HPMCsetTuning(struct HPMCHistoPyramid* h, int32_t classify_waves, int32_t zseg, int32_t emit_workgroups)
+{
+    if (!h) return fail(nullptr, HPMC_ERR_NULL, "h == NULL");
+    if (classify_waves < 0 || classify_waves > hpmc::kClassifyMaxWaves || (zseg && zseg != 8 && zseg != 16 &&
+                                                                           zseg != 32) || emit_workgroups < 0)
+        return fail(h, HPMC_ERR_ARG, "invalid tuning (waves 0..16, zseg 0/8/16/32, workgroups >= 0)");
+    h->tuning.classify_waves = classify_waves;
+    h->tuning.zseg = zseg;
+    h->tuning.emit_wgs = emit_workgroups;
+    h->tainted = true;
+    ok(h);
+    return HPMC_OK;
+}
+
+int32_t HPMCgetTuning(struct HPMCHistoPyramid* h, int32_t out[3])
+{
+    if (!h || !out) return fail(h, HPMC_ERR_NULL, "NULL argument");
+    if (h->tainted && !setup(h)) return h->last_error;
+    out[0] = h->plan.cls_waves;
+    out[1] = h->plan.vols[0].g.zseg;
+    out[2] = h->plan.emit_wgs;
     ok(h);
     return HPMC_OK;
 }

@@ -159,6 +159,15 @@ const char* HPMCgetLastErrorString(struct HPMCHistoPyramid* h);
 /* HIP stream (hipStream_t) for all device work of h; NULL = null stream. */
 int32_t HPMCsetStream(struct HPMCHistoPyramid* h, void* hip_stream);
 
+/* Launch shapes of h's kernels, as HPMCbatchSetTuning; 0 picks
+ * automatically.  Results do not depend on them.  Takes effect at the next
+ * build (a reconfiguration).  No counterpart in the reference, whose launch
+ * shapes are fixed by the GL pipeline. */
+int32_t HPMCsetTuning(struct HPMCHistoPyramid* h, int32_t classify_waves, int32_t zseg, int32_t emit_workgroups);
+/* The shapes in use: {classify_waves, zseg, emit_workgroups} (configures h
+ * if needed). */
+int32_t HPMCgetTuning(struct HPMCHistoPyramid* h, int32_t out[3]);
+
 /* Sets a __device__ global of the field snippet, by name, from host memory
  * (takes the place of glUniform* on HPMCgetBuilderProgram's program,
  * apps/metaballs/metaballs.cpp:277-291).  Stream-ordered with builds. */

@@ -83,3 +83,25 @@ def test_gradient_snippet_cayley():
     f4 = workloads.cayley_grad(lattice, extra=2)
     f = np.ascontiguousarray(f4[..., 3])
     assert check(res, f, 0.0, lattice, clamp=False, mode="gradient", field4=f4) > 10000
+
+
+@pytest.mark.parametrize("tuning", [(1, 8, 8), (4, 16, 256), (2, 32, 0)])
+def test_launch_shapes_do_not_change_results(tuning):
+    # HPMCsetTuning: launch shapes of one HistoPyramid; every shape gives the
+    # same codes, count, (cell, edge) list and vertex bits
+    lattice = (150, 70, 90)
+    f = smooth_noise(lattice, 17)
+    vol = torch.from_numpy(f).cuda()
+    ref = IsoSurface(lattice)
+    ref.set_volume(vol)
+    v0, ce0 = ref.extract_all(0.5, with_cell_edge=True)
+    iso = IsoSurface(lattice)
+    iso.set_volume(vol)
+    iso.set_tuning(*tuning)
+    v1, ce1 = iso.extract_all(0.5, with_cell_edge=True)
+    got = iso.tuning()
+    assert got[0] == tuning[0] and got[1] == tuning[1]
+    assert torch.equal(ce0, ce1) and torch.equal(v0, v1)
+    assert torch.equal(ref.cell_codes(), iso.cell_codes())
+    ref.close()
+    iso.close()
