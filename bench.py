@@ -179,6 +179,53 @@ def copy_calibration(steps):
                       "rw_GBps": 2 * x.numel() * 4 / (ms * 1e-3) / 1e9}))
 
 
+def measure_pipelined(n, steps, warmup, inflight=3):
+    """Frames in flight: `inflight` HistoPyramids on their own HIP streams,
+    each over its own copy of the volume (no cache reuse between frames),
+    called round robin -- frame k's extraction overlaps frame k+1's
+    classification.  Every frame is a full build + extract.  Returns the
+    steady-state time per frame (median of 5 runs of `steps` frames)."""
+    import torch
+    import hpmc_amd
+    dev = torch.device("cuda", 0)
+    base = cayley_volume_torch(n, dev)
+    objs = []
+    for _ in range(inflight):
+        s = torch.cuda.Stream(device=dev)
+        vol = base.clone()
+        iso = hpmc_amd.IsoSurface((n, n, n), stream=s)
+        iso.set_volume(vol)
+        iso.build(0.0)
+        nv = iso.count()
+        out = torch.empty((nv, 6), dtype=torch.float32, device=dev)
+        iso.bind_output(out)
+        objs.append((iso, vol, out, s))
+    del base
+
+    def frames(k):
+        for i in range(k):
+            objs[i % inflight][0].build(0.0)
+            objs[i % inflight][0].extract()
+
+    frames(warmup + inflight)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        frames(steps)
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3 / steps)
+    counts = [o[0].count() for o in objs]
+    assert len(set(counts)) == 1, counts
+    for o in objs:
+        o[0].close()
+    ms = sorted(ts)[2]
+    return {"inflight": inflight, "streams": inflight, "ms_per_frame": ms,
+            "cells_per_s": (n - 1) ** 3 / (ms * 1e-3),
+            "note": "each frame a full build + extract over its own volume buffer; "
+                    "steady-state interval, not the latency of one frame"}
+
+
 def traffic():
     """HBM bytes per classify launch from the committed counter profile of this
     workload, or None when the profile is absent."""
@@ -200,6 +247,7 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched/plain comparison")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--no-pipeline", action="store_true", help="skip the frames-in-flight measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 512 MiB copy instead (counter calibration)")
     args = ap.parse_args()
@@ -286,6 +334,10 @@ def main():
                      "traffic": traffic() if n == 512 else None, "algorithmic_bytes": lattice_bytes},
     }
     iso.close()
+    if not args.no_pipeline:
+        pl = measure_pipelined(n, args.steps, args.warmup)
+        pl["hbm_frac"] = algo_bytes / (pl["ms_per_frame"] * 1e-3) / (HBM_PEAK_GBS * 1e9)
+        line["pipelined"] = pl
     if not args.no_batch:
         line["batched"] = measure_batched(args.batch_reps)
     if not args.no_cpu_baseline:
