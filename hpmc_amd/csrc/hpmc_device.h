@@ -668,10 +668,10 @@ __device__ __forceinline__ void scan_load_tile(const VolDesc& d, const uint16_t*
 // items -- one per kEmitChunk vertices of a block -- in (volume, z segment,
 // y, x, z) order, so that consecutive items are consecutive slices of one
 // block column.  Tile 0 writes the totals, to device memory and to the
-// host-visible copy.  The last workgroup to finish clears the tile sums for
-// the next build.
+// host-visible copy.  The last workgroup to have read the tile sums clears
+// them for the next build.
 struct ScanState {
-    uint32_t done;            // workgroups finished
+    uint32_t done;            // workgroups done with the tile sums
     uint32_t pad[3];
 };
 
@@ -696,7 +696,8 @@ __device__ __forceinline__ void scan_body(
         pre.x += v.x;
         pre.y += v.y;
     }
-    const uint32_t vol = __builtin_amdgcn_readfirstlane(tile_vol[tile]);
+    // tile_vol == nullptr: a plan of one volume
+    const uint32_t vol = tile_vol ? __builtin_amdgcn_readfirstlane(tile_vol[tile]) : 0u;
     const VolDesc d = vols[vol];
     const uint32_t local = is_col ? tile - n_l4t - d.col_tile0 : tile - d.l4_tile0;
     uint32_t c[kScanPer];
@@ -740,6 +741,11 @@ __device__ __forceinline__ void scan_body(
             if (host_totals) *host_totals = t;
         }
     }
+    // This workgroup has read every tile sum it needs (the reductions above
+    // consumed them): count it out now, so that the round trip overlaps the
+    // scan.  The last workgroup to do so clears the sums for the next build.
+    bool last = false;
+    if (threadIdx.x == 0) last = atomicAdd(&state->done, 1u) == ntiles - 1;
     uint2 inc = mine;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -780,9 +786,7 @@ __device__ __forceinline__ void scan_body(
             for (uint32_t ch = 0; ch * kEmitChunk < c[k]; ++ch) items[off.y++] = make_uint2(key, yv | ch);
         }
     }
-    // the last workgroup to finish clears the tile sums for the next build
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(&state->done, 1u) == ntiles - 1;
+    if (threadIdx.x == 0) s_last = last;
     __syncthreads();
     if (s_last) {
         for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) tile_sums[i] = make_uint2(0, 0);
@@ -905,11 +909,12 @@ __device__ __forceinline__ void emit_store(EmitWaveShared& ws, int zfirst, int n
 
 // BATCH: items of several volumes (the volume of each item is looked up);
 // otherwise every item belongs to vols[0].  mk(d) makes the field of
-// volume descriptor d.
+// volume descriptor d.  nitems: the item total of the scan (the kernel loads
+// it before its LDS set-up, so that the two round trips overlap).
 template <int MODE, bool BATCH, class MK>
 __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, const MK& mk, float T,
                                           const uint2* __restrict__ items, const uint32_t* __restrict__ l4_off,
-                                          const Totals* __restrict__ totals,
+                                          const uint32_t nitems,
                                           float* __restrict__ out, CellEdge* __restrict__ out_ce,
                                           uint32_t capacity, const float4* __restrict__ bnorm, EmitShared& sh,
                                           const AdjointIO adj = AdjointIO{})
@@ -917,7 +922,6 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     constexpr int NP = emit_planes<MODE>();
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const uint32_t nitems = load_uniform(&totals->items);
     const uint32_t nwg = gridDim.x;                 // a multiple of 8
     // workgroups are dealt to the 8 XCDs round robin; give each XCD a
     // contiguous run of logical workgroup indices, and each wave a
@@ -1277,12 +1281,13 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
         __shared__ uint32_t xch[2][hpmc::kClassifyMaxWaves];                                   \
         __shared__ uint16_t stage[hpmc::kClassifyMaxWaves][4][hpmc::kColStride];               \
         __shared__ uint2 red[hpmc::kClassifyMaxWaves][hpmc::kColStride];                       \
-        for (int i = threadIdx.x; i < 256; i += blockDim.x) lut[i] = (uint8_t)(cases[i] & 0xfu); \
-        __syncthreads();                                                                      \
         /* grid padded to a multiple of 8: XCD x takes logical run x */                       \
         const uint32_t t = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);          \
         if (t >= n_wg) return;                                                                \
-        const hpmc::VolDesc dd = vols[__builtin_amdgcn_readfirstlane(cls_vol[t])];            \
+        /* cls_vol == nullptr: a plan of one volume; loaded before the LDS set-up */           \
+        const hpmc::VolDesc dd = vols[cls_vol ? __builtin_amdgcn_readfirstlane(cls_vol[t]) : 0u]; \
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) lut[i] = (uint8_t)(cases[i] & 0xfu); \
+        __syncthreads();                                                                      \
         hpmc::classify_body(MAKE_FIELD, dd.g, T, colcnt + dd.colcnt_off, dd.g.zseg, lut, xch, \
                             stage, (int)(t - dd.cls_wg0), red, tile_sums + dd.l4_tile0,       \
                             tile_sums + n_l4t + dd.col_tile0);                                \
@@ -1302,11 +1307,12 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
         const uint64_t* cases, const float4* bnorm)                                           \
     {                                                                                         \
         __shared__ hpmc::EmitShared sh;                                                       \
+        const uint32_t nitems = hpmc::load_uniform(&totals->items);                           \
         sh.cases[threadIdx.x] = cases[threadIdx.x];                                           \
         sh.lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);                           \
         __syncthreads();                                                                      \
         auto mk = [](const hpmc::VolDesc& dd) { return MAKE_FIELD; };                         \
-        hpmc::emit_body<MODE, BATCH>(vols, mk, T, items, l4_off, totals, out, out_ce,         \
+        hpmc::emit_body<MODE, BATCH>(vols, mk, T, items, l4_off, nitems, out, out_ce,         \
                                      capacity, bnorm, sh);                                    \
     }
 
@@ -1318,11 +1324,12 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
         uint32_t* keys, float* vals, float* dT)                                               \
     {                                                                                         \
         __shared__ hpmc::EmitShared sh;                                                       \
+        const uint32_t nitems = hpmc::load_uniform(&totals->items);                           \
         sh.cases[threadIdx.x] = cases[threadIdx.x];                                           \
         sh.lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);                           \
         __syncthreads();                                                                      \
         auto mk = [](const hpmc::VolDesc& dd) { return MAKE_FIELD; };                         \
-        hpmc::emit_body<hpmc::kEmitAdjoint, BATCH>(vols, mk, T, items, l4_off, totals, nullptr, \
+        hpmc::emit_body<hpmc::kEmitAdjoint, BATCH>(vols, mk, T, items, l4_off, nitems, nullptr, \
                                                    nullptr, count, nullptr, sh,               \
                                                    hpmc::AdjointIO{gout, keys, vals, dT});    \
     }

@@ -348,7 +348,7 @@ inline hipError_t launch(const Kernel& k, unsigned grid, unsigned threads, void*
 inline hipError_t plan_classify(const Plan& p, const Kernel& k, float T, const uint64_t* cases, hipStream_t stream)
 {
     const hpmc::VolDesc* vols = p.d_vols;
-    const uint32_t* cv = p.d_cls_vol;
+    const uint32_t* cv = p.nvol == 1 ? nullptr : p.d_cls_vol;   // one volume: no lookup
     uint32_t n = p.n_wg;
     float Ta = T;
     uint16_t* cc = p.colcnt;
@@ -365,7 +365,7 @@ inline hipError_t plan_scan(const Plan& p, const void* k_scan, hipStream_t strea
 {
     const unsigned grid = p.n_l4t + p.n_colt;
     const hpmc::VolDesc* vols = p.d_vols;
-    const uint32_t* tv = p.d_tile_vol;
+    const uint32_t* tv = p.nvol == 1 ? nullptr : p.d_tile_vol;
     uint32_t nl = p.n_l4t, nc = p.n_colt;
     const uint16_t* cc = p.colcnt;
     uint2* ts = p.tile_sums;
