@@ -52,6 +52,9 @@ struct Geom {
     float ipx, ipy, ipz;  // 1 / lattice size (snippet texel-centre coordinates)
     int zseg;             // classification slices per workgroup: 8, 16 or 32
     int nbx, nby, npz;    // level-4 block columns along x, y; z segments
+    // tsx2, tsy2, tiles_x and zseg are powers of two: their log2, so that
+    // key and slice arithmetic is shifts and masks
+    int tsx2_log2, tsy2_log2, tiles_x_log2, zseg_log2;
 };
 
 // Per (z segment, y, x) block column the vertex counts of its slices, 32
@@ -137,9 +140,9 @@ __device__ __forceinline__ uint32_t morton_compact(uint32_t v)
 // Level-4 block key of the 16x16 cell block starting at (x0, y0) of slice z.
 __device__ __forceinline__ uint32_t block_key(const Geom& g, int x0, int y0, int z)
 {
-    uint32_t tx = (uint32_t)(z % g.tiles_x), ty = (uint32_t)(z / g.tiles_x);
-    uint32_t X = (tx * (uint32_t)g.tsx2 + (uint32_t)x0) >> kBlockLog2;
-    uint32_t Y = (ty * (uint32_t)g.tsy2 + (uint32_t)y0) >> kBlockLog2;
+    const uint32_t tx = (uint32_t)z & ((1u << g.tiles_x_log2) - 1u), ty = (uint32_t)z >> g.tiles_x_log2;
+    const uint32_t X = ((tx << g.tsx2_log2) + (uint32_t)x0) >> kBlockLog2;
+    const uint32_t Y = ((ty << g.tsy2_log2) + (uint32_t)y0) >> kBlockLog2;
     return morton_spread(X) | (morton_spread(Y) << 1);
 }
 
@@ -149,11 +152,11 @@ __device__ __forceinline__ uint32_t key_count(const Geom& g, const uint16_t* __r
 {
     const int X = (int)morton_compact(key) << kBlockLog2;
     const int Y = (int)morton_compact(key >> 1) << kBlockLog2;
-    const int tx = X / g.tsx2, ty = Y / g.tsy2;
-    const int z = ty * g.tiles_x + tx;
-    const int bx = (X - tx * g.tsx2) >> kBlockLog2, by = (Y - ty * g.tsy2) >> kBlockLog2;
+    const int tx = X >> g.tsx2_log2, ty = Y >> g.tsy2_log2;
+    const int z = (ty << g.tiles_x_log2) + tx;
+    const int bx = (X & (g.tsx2 - 1)) >> kBlockLog2, by = (Y & (g.tsy2 - 1)) >> kBlockLog2;
     if (tx >= g.tiles_x || z >= g.cz || bx >= g.nbx || by >= g.nby) return 0u;
-    const int pz = z / g.zseg, zi = z - pz * g.zseg;
+    const int pz = z >> g.zseg_log2, zi = z & (g.zseg - 1);
     return colcnt[(((size_t)pz * g.nby + by) * g.nbx + bx) * kColStride + zi];
 }
 
@@ -771,16 +774,17 @@ __device__ __forceinline__ void scan_body(
         }
     } else {
         const Geom& g = d.g;
+        // a thread's 16 entries lie in one 32-slot block column
+        static_assert(kColStride % kScanPer == 0, "column entries of a thread span one column");
+        const uint32_t wi = base / (uint32_t)kColStride;
+        const int bxb = (int)(wi % (uint32_t)g.nbx);
+        const uint32_t rest = wi / (uint32_t)g.nbx;
+        const int by = (int)(rest % (uint32_t)g.nby);
+        const int pz = (int)(rest / (uint32_t)g.nby);
 #pragma unroll
         for (int k = 0; k < kScanPer; ++k) {
             if (!c[k]) continue;
-            const uint32_t ci = base + k;
-            const uint32_t wi = ci / (uint32_t)kColStride;
-            const int zi = (int)(ci % (uint32_t)kColStride);
-            const int bxb = (int)(wi % (uint32_t)g.nbx);
-            const uint32_t rest = wi / (uint32_t)g.nbx;
-            const int by = (int)(rest % (uint32_t)g.nby);
-            const int pz = (int)(rest / (uint32_t)g.nby);
+            const int zi = (int)(base % (uint32_t)kColStride) + k;
             const uint32_t key = block_key(g, bxb << kBlockLog2, by << kBlockLog2, pz * g.zseg + zi);
             const uint32_t yv = (c[k] << 4) | (vol << 16);
             for (uint32_t ch = 0; ch * kEmitChunk < c[k]; ++ch) items[off.y++] = make_uint2(key, yv | ch);
@@ -862,10 +866,10 @@ __device__ __forceinline__ void block_origin(const Geom& g, uint32_t key, int& x
 {
     const int X0 = (int)morton_compact(key) << kBlockLog2;
     const int Y0 = (int)morton_compact(key >> 1) << kBlockLog2;
-    const int tx = X0 / g.tsx2, ty = Y0 / g.tsy2;
-    z = ty * g.tiles_x + tx;
-    x0 = X0 - tx * g.tsx2;
-    y0 = Y0 - ty * g.tsy2;
+    const int tx = X0 >> g.tsx2_log2, ty = Y0 >> g.tsy2_log2;
+    z = (ty << g.tiles_x_log2) + tx;
+    x0 = X0 & (g.tsx2 - 1);
+    y0 = Y0 & (g.tsy2 - 1);
 }
 
 // Loads planes p_lo..p_hi (1 to 3) of the 18x18 window at (x0, y0) into r.

@@ -133,6 +133,15 @@ inline hpmc::Geom make_geom(const VolumeSpec& v, const Layout& lay, int zseg)
     g.nbx = (g.cx + hpmc::kBlock - 1) / hpmc::kBlock;
     g.nby = (g.cy + hpmc::kBlock - 1) / hpmc::kBlock;
     g.npz = (g.cz + zseg - 1) / zseg;
+    auto log2_of = [](int v) {
+        int l = 0;
+        while ((1 << l) < v) ++l;
+        return l;
+    };
+    g.tsx2_log2 = log2_of(g.tsx2);
+    g.tsy2_log2 = log2_of(g.tsy2);
+    g.tiles_x_log2 = log2_of(g.tiles_x);
+    g.zseg_log2 = log2_of(zseg);
     return g;
 }
 
