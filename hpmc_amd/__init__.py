@@ -24,6 +24,9 @@ ERRORS = {
     7: "HPMC_ERR_CAPACITY", 8: "HPMC_ERR_ARG", 9: "HPMC_ERR_UNSUPPORTED",
 }
 
+# bytes of hpmc::TraversalView (HPMC_TRAVERSAL_VIEW_BYTES of include/hpmc.h)
+TRAVERSAL_VIEW_BYTES = 64
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _u32 = ctypes.c_uint32
@@ -52,6 +55,8 @@ SIGNATURES = [
     ("HPMCdestroyTraversalHandle", None, [_vp]),
     ("HPMCgetTraversalShaderFunctions", _vp, [_vp]),
     ("HPMCsetTraversalHandleProgram", _b, [_vp, _u32, _u32, _u32, _u32]),
+    ("HPMCgetTraversalView", _i32, [_vp, _vp, ctypes.c_size_t]),
+    ("HPMCgetTraversalGrid", _i32, [_vp]),
     ("HPMCextractVertices", _b, [_vp]),
     ("HPMCextractVerticesTransformFeedback", _b, [_vp]),
     ("HPMCextractVerticesTransformFeedbackNV", _b, [_vp]),
@@ -129,6 +134,14 @@ def check_snippet(source, gradient=False, binary=False):
     code = lib().HPMCcheckFieldSnippet(source.encode(), flags, buf, len(buf))
     _check(code)
     return buf.value.decode()
+
+
+def _libc_free(p):
+    """free() for strings the library hands out with malloc."""
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [_vp]
+    libc.free.restype = None
+    libc.free(p)
 
 
 def _stream_ptr(stream):
@@ -222,6 +235,32 @@ class IsoSurface:
             if not self.th:
                 raise HPMCError("HPMCcreateTraversalHandle failed: %s" % (last_error(self.h)[1],))
         return self.th
+
+    def traversal_source(self):
+        """hipRTC source for kernels that fuse the traversal with their own
+        per-vertex work (defines hpmc::TraversalView and
+        hpmc_for_each_vertex)."""
+        L = lib()
+        p = L.HPMCgetTraversalShaderFunctions(self._traversal())
+        if not p:
+            raise HPMCError("HPMCgetTraversalShaderFunctions failed: %s" % (last_error(self.h)[1],))
+        try:
+            return ctypes.string_at(p).decode()
+        finally:
+            _libc_free(p)
+
+    def traversal_view(self):
+        """The 64-byte hpmc::TraversalView of the last build (bytes)."""
+        buf = ctypes.create_string_buffer(TRAVERSAL_VIEW_BYTES)
+        _check(lib().HPMCgetTraversalView(self._traversal(), buf, TRAVERSAL_VIEW_BYTES), self.h)
+        return buf.raw
+
+    def traversal_grid(self):
+        """Workgroups of 256 threads for a traversal kernel."""
+        n = int(lib().HPMCgetTraversalGrid(self._traversal()))
+        if n <= 0:
+            raise HPMCError("HPMCgetTraversalGrid failed: %s" % (last_error(self.h)[1],))
+        return n
 
     def bind_output(self, out, cell_edge=None):
         """out: float32 device tensor (capacity, 6); cell_edge: int32 (capacity, 2) or None."""

@@ -375,14 +375,11 @@ bool determine_layout(HPMCHistoPyramid* h)
     return true;
 }
 
-std::string rtc_source(const std::string& snippet, bool gradient, bool binary)
+// The snippet and the field object built on it (hpmc::SnippetField), for
+// the hipRTC sources that need them.
+std::string snippet_field_source(const std::string& snippet)
 {
     std::string s;
-    s += "#define HPMC_RTC 1\n";
-    if (gradient) s += "#define HPMC_FIELD_GRADIENT 1\n";
-    s += "typedef unsigned char uint8_t;\ntypedef unsigned short uint16_t;\n";
-    s += "typedef unsigned int uint32_t;\ntypedef unsigned long long uint64_t;\n";
-    s += "#include \"hpmc_device.h\"\n";
     s += "#line 1 \"field_snippet\"\n";
     s += snippet;
     s += "\n#line 1 \"hpmc_rtc_tail\"\n";
@@ -414,8 +411,20 @@ std::string rtc_source(const std::string& snippet, bool gradient, bool binary)
          "    }\n"
          "#endif\n"
          "};\n"
-         "}\n"
-         "HPMC_DEFINE_BUILD_KERNELS(rtc, (hpmc::SnippetField{dd.g.ipx, dd.g.ipy, dd.g.ipz}))\n";
+         "}\n";
+    return s;
+}
+
+std::string rtc_source(const std::string& snippet, bool gradient, bool binary)
+{
+    std::string s;
+    s += "#define HPMC_RTC 1\n";
+    if (gradient) s += "#define HPMC_FIELD_GRADIENT 1\n";
+    s += "typedef unsigned char uint8_t;\ntypedef unsigned short uint16_t;\n";
+    s += "typedef unsigned int uint32_t;\ntypedef unsigned long long uint64_t;\n";
+    s += "#include \"hpmc_device.h\"\n";
+    s += snippet_field_source(snippet);
+    s += "HPMC_DEFINE_BUILD_KERNELS(rtc, (hpmc::SnippetField{dd.g.ipx, dd.g.ipy, dd.g.ipz}))\n";
     const char* mode = binary ? "hpmc::kEmitBinary" : gradient ? "hpmc::kEmitGradient" : "hpmc::kEmitContinuous";
     s += std::string("HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_rtc, (hpmc::SnippetField{dd.g.ipx, dd.g.ipy, dd.g.ipz}), ") +
          mode + ", false)\n";
@@ -820,7 +829,11 @@ char* HPMCgetTraversalShaderFunctions(struct HPMCTraversalHandle* th)
         return nullptr;
     }
     HPMCHistoPyramid* h = th->h;
-    if (h->tainted && !setup(h)) return nullptr;
+    // the configuration as the next build takes it (no device work here)
+    if (!determine_layout(h)) return nullptr;
+    const bool binary = h->tainted ? h->binary : h->binary_active;
+    // One self-contained hipRTC source: the layout, the engine's device code,
+    // this HistoPyramid's field and emission mode, and hpmc_for_each_vertex.
     char defs[1024];
     snprintf(defs, sizeof defs,
              "// HPMC layout of this HistoPyramid (hpmc/src/shadergen.cpp:45-83)\n"
@@ -830,13 +843,76 @@ char* HPMCgetTraversalShaderFunctions(struct HPMCTraversalHandle* th)
              "#define HPMC_TILE_SIZE_X %d\n#define HPMC_TILE_SIZE_Y %d\n#define HPMC_HP_SIZE_L2 %d\n",
              h->size[0], h->size[1], h->size[2], h->cells[0], h->cells[1], h->cells[2], h->tiles[0], h->tiles[1],
              h->tile[0], h->tile[1], h->l2);
+    const bool snippet = h->mode == kFieldCustom;
     std::string s = defs;
-    s += kEmbeddedTablesHeader;
-    s += kEmbeddedDeviceHeader;
+    s += "#define HPMC_RTC 1\n#define HPMC_TABLES_INLINE 1\n";
+    if (h->gradient) s += "#define HPMC_FIELD_GRADIENT 1\n";
+    s += "typedef unsigned char uint8_t;\ntypedef unsigned short uint16_t;\n";
+    s += "typedef unsigned int uint32_t;\ntypedef unsigned long long uint64_t;\n";
+    // the headers inline, without their include guards' pragma (one file)
+    auto inline_header = [](const char* text) {
+        std::string t = text;
+        for (size_t at; (at = t.find("#pragma once")) != std::string::npos;) t.erase(at, 12);
+        return t + "\n";
+    };
+    s += inline_header(kEmbeddedTablesHeader);
+    s += inline_header(kEmbeddedDeviceHeader);
+    if (snippet) s += snippet_field_source(h->snippet);
+    s += "#line 1 \"hpmc_traversal\"\n";
+    if (snippet)
+        s += "#define HPMC_TRAVERSAL_FIELD(dd) (hpmc::SnippetField{(dd).g.ipx, (dd).g.ipy, (dd).g.ipz})\n";
+    else if (h->gradient)
+        s += "#define HPMC_TRAVERSAL_FIELD(dd) "
+             "(hpmc::Volume4Field{(const float4*)(dd).field, (dd).g.sx, (dd).g.sy, (dd).g.sz})\n";
+    else
+        s += "#define HPMC_TRAVERSAL_FIELD(dd) "
+             "(hpmc::VolumeField{(const float*)(dd).field, (dd).g.sx, (dd).g.sy, (dd).g.sz})\n";
+    s += std::string("#define HPMC_TRAVERSAL_MODE ") +
+         (binary ? "hpmc::kEmitBinary" : h->gradient ? "hpmc::kEmitGradient" : "hpmc::kEmitContinuous") +
+         "\n";
+    s += "// cb(k, normal, position) for every vertex k of the build the view\n"
+         "// describes (HPMCgetTraversalView); CB needs a static constexpr bool\n"
+         "// kActive = true.  Launch HPMCgetTraversalGrid workgroups of 256 threads.\n"
+         "template <class CB>\n"
+         "__device__ __forceinline__ void hpmc_for_each_vertex(const hpmc::TraversalView& v, const CB& cb)\n"
+         "{\n"
+         "    auto mk = [](const hpmc::VolDesc& dd) { return HPMC_TRAVERSAL_FIELD(dd); };\n"
+         "    hpmc::for_each_vertex<HPMC_TRAVERSAL_MODE>(v, mk, cb);\n"
+         "}\n";
     char* r = (char*)malloc(s.size() + 1);
     if (!r) return nullptr;
     memcpy(r, s.c_str(), s.size() + 1);
     return r;
+}
+
+int32_t HPMCgetTraversalView(struct HPMCTraversalHandle* th, void* view, size_t bytes)
+{
+    if (!th || !th->h || !view) return fail(th ? th->h : nullptr, HPMC_ERR_NULL, "NULL argument");
+    static_assert(sizeof(hpmc::TraversalView) == HPMC_TRAVERSAL_VIEW_BYTES, "view size");
+    HPMCHistoPyramid* h = th->h;
+    if (bytes < sizeof(hpmc::TraversalView))
+        return fail(h, HPMC_ERR_ARG, "view needs %zu bytes", sizeof(hpmc::TraversalView));
+    if (h->broken) return fail(h, HPMC_ERR_BROKEN, "HistoPyramid is broken");
+    if (h->tainted || !h->built) return fail(h, HPMC_ERR_ARG, "HistoPyramid has not been built since it was configured");
+    hpmc::TraversalView v{};
+    v.vols = h->plan.d_vols;
+    v.items = h->plan.items;
+    v.l4_off = h->plan.l4_off;
+    v.totals = h->plan.d_totals;
+    v.cases = h->constants->d_cases;
+    v.bnorm = h->constants->d_bnorm;
+    v.threshold = h->threshold;
+    memcpy(view, &v, sizeof v);
+    ok(h);
+    return HPMC_OK;
+}
+
+int32_t HPMCgetTraversalGrid(struct HPMCTraversalHandle* th)
+{
+    if (!th || !th->h) return 0;
+    HPMCHistoPyramid* h = th->h;
+    if (h->tainted && !setup(h)) return 0;
+    return h->plan.emit_wgs;
 }
 
 bool HPMCsetTraversalHandleProgram(struct HPMCTraversalHandle* th, uint32_t program, uint32_t u1, uint32_t u2,

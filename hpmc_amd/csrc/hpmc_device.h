@@ -28,7 +28,15 @@
 #include <stdint.h>
 #endif
 
+#ifndef HPMC_TABLES_INLINE
 #include "mc_tables.h"
+#endif
+
+// No implicit fused multiply-add in this code: the kernels are compiled
+// ahead of time and by hipRTC (library snippets, user traversal kernels,
+// where contraction is off), and both must give the same bits; the fused
+// operations the emission wants are written out as fma.
+#pragma clang fp contract(off)
 
 namespace hpmc {
 
@@ -837,6 +845,14 @@ struct AdjointIO {
     float* dT;
 };
 
+// Per-vertex callback of user-fused traversal kernels: emission calls
+// cb(k, normal, position) for output vertex k instead of writing the
+// output buffer.  The default does nothing and leaves emission as it is.
+struct NoVertexCallback {
+    static constexpr bool kActive = false;
+    __device__ void operator()(uint32_t, float3, float3) const {}
+};
+
 // Lattice planes an emit item stages: z, z+1 for the corners, z+2 for the
 // forward differences of continuous mode.
 template <int MODE>
@@ -915,13 +931,13 @@ __device__ __forceinline__ void emit_store(EmitWaveShared& ws, int zfirst, int n
 // otherwise every item belongs to vols[0].  mk(d) makes the field of
 // volume descriptor d.  nitems: the item total of the scan (the kernel loads
 // it before its LDS set-up, so that the two round trips overlap).
-template <int MODE, bool BATCH, class MK>
+template <int MODE, bool BATCH, class MK, class CB = NoVertexCallback>
 __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, const MK& mk, float T,
                                           const uint2* __restrict__ items, const uint32_t* __restrict__ l4_off,
                                           const uint32_t nitems,
                                           float* __restrict__ out, CellEdge* __restrict__ out_ce,
                                           uint32_t capacity, const float4* __restrict__ bnorm, EmitShared& sh,
-                                          const AdjointIO adj = AdjointIO{})
+                                          const AdjointIO adj = AdjointIO{}, const CB& cb = CB{})
 {
     constexpr int NP = emit_planes<MODE>();
     const int lane = threadIdx.x & 63;
@@ -932,7 +948,7 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     // contiguous run of items: consecutive items are mostly the next slice
     // of the same block column (or the next chunk of the same block), whose
     // planes are already staged
-    const uint32_t t = (blockIdx.x & 7u) * (nwg >> 3) + (blockIdx.x >> 3);
+    const uint32_t t = (nwg & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (nwg >> 3) + (blockIdx.x >> 3);
     const uint64_t nwaves = (uint64_t)nwg * kEmitWaves;
     const uint64_t wlog = (uint64_t)t * kEmitWaves + wv;
     uint32_t it = __builtin_amdgcn_readfirstlane((uint32_t)(wlog * nitems / nwaves));
@@ -1082,9 +1098,11 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                 const float4 fb = field.grad(cx0 + bx_, cy0 + by_, cz + bz_);
                 tt = (fa.w - T) * __builtin_amdgcn_rcpf(fa.w - fb.w);
                 const float u = 1.0f - tt;
-                nx = T - (fa.x * u + fb.x * tt);
-                ny = T - (fa.y * u + fb.y * tt);
-                nz = T - (fa.z * u + fb.z * tt);
+                // explicit fma: the same bits however the code is compiled
+                // (ahead of time, or by hipRTC with contraction off)
+                nx = T - __builtin_fmaf(fa.x, u, fb.x * tt);
+                ny = T - __builtin_fmaf(fa.y, u, fb.y * tt);
+                nz = T - __builtin_fmaf(fa.z, u, fb.z * tt);
             } else {
                 // sample (x, y, plane) at x + 18 y + 384 slot of the staged
                 // brick; a and b lie in planes 0 or 1, their +z neighbours in
@@ -1151,9 +1169,19 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                     adj.dT[o] = dT;
                     continue;
                 }
-                nx = T - (nax * u + nbx_ * tt);
-                ny = T - (nay * u + nby_ * tt);
-                nz = T - (naz * u + nbz_ * tt);
+                // explicit fma: the same bits however the code is compiled
+                // (ahead of time, or by hipRTC with contraction off)
+                nx = T - __builtin_fmaf(nax, u, nbx_ * tt);
+                ny = T - __builtin_fmaf(nay, u, nby_ * tt);
+                nz = T - __builtin_fmaf(naz, u, nbz_ * tt);
+            }
+            if constexpr (CB::kActive) {
+                // a user-fused traversal: hand the vertex over, write nothing
+                if (v0 + (uint32_t)lane < vhi)
+                    cb(base + v, make_float3(nx * g.scx, ny * g.scy, nz * g.scz),
+                       make_float3((gx + (axis == 0 ? tt : 0.0f)) * g.scx, (gy + (axis == 1 ? tt : 0.0f)) * g.scy,
+                                   (gz + (axis == 2 ? tt : 0.0f)) * g.scz));
+                continue;
             }
             // The batch's 64 x 24 bytes go out as three contiguous 512-byte
             // stores (lane L writes floats 2(64k + L), +1) after a transpose
@@ -1268,6 +1296,43 @@ __device__ __forceinline__ void codes_body(const F& field, const Geom& g, float 
             code |= (field.at(x + (b & 1), y + ((b >> 1) & 1), z + (b >> 2)) < T ? 1u : 0u) << b;
         codes[c] = (uint8_t)code;
     }
+}
+
+// Device-side view of one HistoPyramid's last build, for kernels that fuse
+// the traversal with their own per-vertex work (the HIP counterpart of the
+// reference's traversal shader functions, triface.cpp:125-192).  Filled on
+// the host by HPMCgetTraversalView; 64 bytes.
+struct TraversalView {
+    const VolDesc* vols;
+    const uint2* items;
+    const uint32_t* l4_off;
+    const Totals* totals;
+    const uint64_t* cases;
+    const float4* bnorm;
+    float threshold;
+    uint32_t reserved[3];
+};
+
+// Visits every vertex of the view's build once: cb(k, normal, position)
+// with k the vertex's index in the reference's extraction order and the
+// same bits the extraction writes.  Call it from every thread of a kernel
+// launched with workgroups of 256 threads (a grid of
+// HPMCgetTraversalGrid workgroups, a multiple of 8, spreads the work as the
+// extraction does; other grid sizes visit the same vertices).  MK makes the
+// field from a volume descriptor, MODE is the emission mode of the build.
+template <int MODE, class MK, class CB>
+__device__ __forceinline__ void for_each_vertex(const TraversalView& v, const MK& mk, const CB& cb)
+{
+    __shared__ EmitShared sh;
+    const uint32_t nitems = load_uniform(&v.totals->items);
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+        sh.cases[i] = v.cases[i];
+        sh.lut[i] = (uint8_t)(v.cases[i] & 0xfu);
+    }
+    __syncthreads();
+    if (blockDim.x != 64 * kEmitWaves) return;   // the emission loop needs 4 waves per workgroup
+    emit_body<MODE, false>(v.vols, mk, v.threshold, v.items, v.l4_off, nitems, nullptr, nullptr, 0xffffffffu,
+                           v.bnorm, sh, AdjointIO{}, cb);
 }
 
 }  // namespace hpmc

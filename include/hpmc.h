@@ -114,6 +114,23 @@ void HPMCdestroyTraversalHandle(struct HPMCTraversalHandle* th);
  * hipRTC kernels.  The caller frees it with free(). */
 char* HPMCgetTraversalShaderFunctions(struct HPMCTraversalHandle* th);
 
+/* Bytes of the traversal view (hpmc::TraversalView in the source above). */
+#define HPMC_TRAVERSAL_VIEW_BYTES 64
+
+/* Device-side view of the last build of th's HistoPyramid for a kernel
+ * compiled from HPMCgetTraversalShaderFunctions' source: pass it by value
+ * to a kernel that calls hpmc_for_each_vertex(view, cb), which calls
+ * cb(k, normal, position) once for every vertex k (the extraction's order
+ * and bits).  Launch HPMCgetTraversalGrid(th) workgroups of 256 threads on
+ * the HistoPyramid's stream after the build.  The view stays valid until
+ * the HistoPyramid is reconfigured; it describes whichever build ran last.
+ * No counterpart in the reference, where the traversal functions read the
+ * HistoPyramid texture bound by HPMCsetTraversalHandleProgram
+ * (triface.cpp:195-327). */
+int32_t HPMCgetTraversalView(struct HPMCTraversalHandle* th, void* view, size_t bytes);
+/* Workgroups (of 256 threads) for a traversal kernel; 0 on error. */
+int32_t HPMCgetTraversalGrid(struct HPMCTraversalHandle* th);
+
 /* Replaces HPMCsetTraversalHandleProgram (hpmc.h:517-522,
  * triface.cpp:195-327).  There is no GL program: the call validates its
  * arguments like the reference (non-zero program, distinct units) and
