@@ -46,6 +46,7 @@ SIGNATURES = [
     ("HPMCsetGridSize", None, [_vp, _i32, _i32, _i32]),
     ("HPMCsetGridExtent", None, [_vp, _f, _f, _f]),
     ("HPMCsetFieldTexture3D", None, [_vp, _vp, _u8]),
+    ("HPMCsetFieldTexture3DUnorm8", None, [_vp, _vp]),
     ("HPMCsetFieldCustom", None, [_vp, ctypes.c_char_p, _u32, _u8]),
     ("HPMCgetBuilderProgram", _u32, [_vp]),
     ("HPMCdestroyHandle", None, [_vp]),
@@ -183,15 +184,20 @@ class IsoSurface:
 
     # -- field ---------------------------------------------------------------
     def set_volume(self, volume, gradient=False):
-        """volume: contiguous float32 device tensor shaped (z, y, x), or with
-        gradient=True (z, y, x, 4) holding (gradient xyz, value) per lattice
-        point like an RGBA gradient texture."""
+        """volume: contiguous device tensor shaped (z, y, x): float32, or
+        uint8 for an 8-bit normalised volume (sample c / 255); or, with
+        gradient=True, float32 (z, y, x, 4) holding (gradient xyz, value) per
+        lattice point like an RGBA gradient texture."""
         import torch
-        assert volume.dtype == torch.float32 and volume.is_contiguous() and volume.is_cuda
+        assert volume.dtype in (torch.float32, torch.uint8) and volume.is_contiguous() and volume.is_cuda
+        assert not (gradient and volume.dtype == torch.uint8), "8-bit volumes have no gradient variant"
         shape = self.lattice[::-1] + ((4,) if gradient else ())
         assert tuple(volume.shape) == shape, (volume.shape, shape)
         self._volume = volume
-        lib().HPMCsetFieldTexture3D(self.h, ctypes.c_void_p(volume.data_ptr()), int(bool(gradient)))
+        if volume.dtype == torch.uint8:
+            lib().HPMCsetFieldTexture3DUnorm8(self.h, ctypes.c_void_p(volume.data_ptr()))
+        else:
+            lib().HPMCsetFieldTexture3D(self.h, ctypes.c_void_p(volume.data_ptr()), int(bool(gradient)))
 
     def set_snippet(self, source, gradient=False):
         """HIP C++ defining HPMC_fetch(float3) (and HPMC_fetchGrad(float3) ->

@@ -33,12 +33,16 @@
 // value); snippet fields get the same kernels from hipRTC.
 #define HPMC_VOLUME_FIELD (hpmc::VolumeField{(const float*)dd.field, dd.g.sx, dd.g.sy, dd.g.sz})
 #define HPMC_VOLUME4_FIELD (hpmc::Volume4Field{(const float4*)dd.field, dd.g.sx, dd.g.sy, dd.g.sz})
+#define HPMC_VOLUME8_FIELD (hpmc::Volume8Field{(const uint8_t*)dd.field, dd.g.sx, dd.g.sy, dd.g.sz})
 HPMC_DEFINE_BUILD_KERNELS(volume, HPMC_VOLUME_FIELD)
 HPMC_DEFINE_BUILD_KERNELS(volume4, HPMC_VOLUME4_FIELD)
+HPMC_DEFINE_BUILD_KERNELS(volume8, HPMC_VOLUME8_FIELD)
 HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume, HPMC_VOLUME_FIELD, hpmc::kEmitContinuous, false)
 HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume_binary, HPMC_VOLUME_FIELD, hpmc::kEmitBinary, false)
 HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume4_gradient, HPMC_VOLUME4_FIELD, hpmc::kEmitGradient, false)
 HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume4_binary, HPMC_VOLUME4_FIELD, hpmc::kEmitBinary, false)
+HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume8, HPMC_VOLUME8_FIELD, hpmc::kEmitContinuous, false)
+HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume8_binary, HPMC_VOLUME8_FIELD, hpmc::kEmitBinary, false)
 HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume_batch, HPMC_VOLUME_FIELD, hpmc::kEmitContinuous, true)
 
 extern "C" __global__ __launch_bounds__(256) void hpmc_scan(
@@ -270,6 +274,8 @@ struct HPMCHistoPyramid {
     bool binary_active = false;      // as taken at the last configuration (setup)
     int mode = kFieldVolume;
     const float* volume = nullptr;
+    bool unorm8 = false;             // the volume is 8-bit normalised (volume8)
+    const uint8_t* volume8 = nullptr;
     bool gradient = false;
     std::string snippet;
     // layout (init.cpp:105-130)
@@ -375,6 +381,21 @@ bool determine_layout(HPMCHistoPyramid* h)
     return true;
 }
 
+// Device pointer of a volume field (FP32, RGBA FP32 or 8-bit).
+const void* volume_ptr(const HPMCHistoPyramid* h)
+{
+    return h->unorm8 ? (const void*)h->volume8 : (const void*)h->volume;
+}
+
+// The classification threshold of an 8-bit volume: the smallest byte c with
+// c / 255 >= T (256 if none), so that c < kT exactly when c / 255 < T.
+float unorm8_threshold(float T)
+{
+    for (int c = 0; c < 256; ++c)
+        if (!((float)c / 255.0f < T)) return (float)c;
+    return 256.0f;
+}
+
 // The snippet and the field object built on it (hpmc::SnippetField), for
 // the hipRTC sources that need them.
 std::string snippet_field_source(const std::string& snippet)
@@ -404,6 +425,7 @@ std::string snippet_field_source(const std::string& snippet)
          "    __device__ __forceinline__ float fetch2(Plane pz, uint32_t col, uint32_t row) const {\n"
          "        return HPMC_fetch(make_float3(((float)col + 0.5f) * ipx, ((float)row + 0.5f) * ipy, pz));\n"
          "    }\n"
+         "    __device__ __forceinline__ float sample(Plane pz, uint32_t idx) const { return fetch(pz, idx); }\n"
          "#ifdef HPMC_FIELD_GRADIENT\n"
          "    __device__ __forceinline__ float4 grad(int i, int j, int k) const {\n"
          "        return HPMC_fetchGrad(make_float3(((float)i + 0.5f) * ipx, ((float)j + 0.5f) * ipy,\n"
@@ -507,7 +529,7 @@ bool setup(HPMCHistoPyramid* h)
     if (!h->tainted) return true;
     free_device(h);
     if (!determine_layout(h)) return false;
-    if (h->mode == kFieldVolume && !h->volume) {
+    if (h->mode == kFieldVolume && !volume_ptr(h)) {
         fail(h, HPMC_ERR_FIELD, "no field volume set");
         return false;
     }
@@ -519,7 +541,12 @@ bool setup(HPMCHistoPyramid* h)
     // when the programs are next generated, i.e. here
     h->binary_active = h->binary;
     if (h->mode == kFieldVolume) {
-        if (h->gradient) {
+        if (h->unorm8) {
+            h->k_classify.builtin = (const void*)hpmc_classify_volume8;
+            h->k_codes.builtin = (const void*)hpmc_codes_volume8;
+            h->k_emit.builtin = h->binary_active ? (const void*)hpmc_emit_volume8_binary
+                                                 : (const void*)hpmc_emit_volume8;
+        } else if (h->gradient) {
             h->k_classify.builtin = (const void*)hpmc_classify_volume4;
             h->k_codes.builtin = (const void*)hpmc_codes_volume4;
             h->k_emit.builtin = h->binary_active ? (const void*)hpmc_emit_volume4_binary
@@ -537,7 +564,7 @@ bool setup(HPMCHistoPyramid* h)
         spec.cells[a] = h->cells[a];
         spec.extent[a] = h->extent[a];
     }
-    spec.field = h->mode == kFieldVolume ? (const void*)h->volume : nullptr;
+    spec.field = h->mode == kFieldVolume ? volume_ptr(h) : nullptr;
     std::string err;
     const int32_t rc = hpmc_host::plan_create(h->plan, {spec}, h->tuning,
                                               (const void*)hpmc_classify_volume, device_shape(), &err);
@@ -567,9 +594,11 @@ void stage_mark(HPMCHistoPyramid* h, int stage)
 
 bool launch_classify(HPMCHistoPyramid* h, float T)
 {
-    if (h->mode == kFieldVolume) HIP_TRY(h, hpmc_host::plan_set_field(h->plan, 0, h->volume, h->stream));
+    if (h->mode == kFieldVolume) HIP_TRY(h, hpmc_host::plan_set_field(h->plan, 0, volume_ptr(h), h->stream));
     stage_mark(h, kStageClassify);
-    HIP_TRY(h, hpmc_host::plan_classify(h->plan, h->k_classify, T, h->constants->d_cases, h->stream));
+    // 8-bit volumes classify raw bytes against the equivalent byte threshold
+    const float Tc = h->mode == kFieldVolume && h->unorm8 ? unorm8_threshold(T) : T;
+    HIP_TRY(h, hpmc_host::plan_classify(h->plan, h->k_classify, Tc, h->constants->d_cases, h->stream));
     HIP_TRY(h, hipGetLastError());
     stage_mark(h, kStageClassify);
     return true;
@@ -734,9 +763,24 @@ void HPMCsetFieldTexture3D(struct HPMCHistoPyramid* h, const float* volume, uint
     // the programs; a change of mode or gradient does.
     h->volume = volume;
     const bool grad = gradient != 0;
-    if (h->mode != kFieldVolume || h->gradient != grad) {
+    if (h->mode != kFieldVolume || h->gradient != grad || h->unorm8) {
         h->mode = kFieldVolume;
         h->gradient = grad;
+        h->unorm8 = false;
+        h->tainted = true;
+        h->broken = false;
+    }
+}
+
+void HPMCsetFieldTexture3DUnorm8(struct HPMCHistoPyramid* h, const uint8_t* volume)
+{
+    if (!h) return;
+    // as HPMCsetFieldTexture3D: a new volume alone does not force a rebuild
+    h->volume8 = volume;
+    if (h->mode != kFieldVolume || h->gradient || !h->unorm8) {
+        h->mode = kFieldVolume;
+        h->gradient = false;
+        h->unorm8 = true;
         h->tainted = true;
         h->broken = false;
     }
@@ -861,6 +905,9 @@ char* HPMCgetTraversalShaderFunctions(struct HPMCTraversalHandle* th)
     s += "#line 1 \"hpmc_traversal\"\n";
     if (snippet)
         s += "#define HPMC_TRAVERSAL_FIELD(dd) (hpmc::SnippetField{(dd).g.ipx, (dd).g.ipy, (dd).g.ipz})\n";
+    else if (h->unorm8)
+        s += "#define HPMC_TRAVERSAL_FIELD(dd) "
+             "(hpmc::Volume8Field{(const uint8_t*)(dd).field, (dd).g.sx, (dd).g.sy, (dd).g.sz})\n";
     else if (h->gradient)
         s += "#define HPMC_TRAVERSAL_FIELD(dd) "
              "(hpmc::Volume4Field{(const float4*)(dd).field, (dd).g.sx, (dd).g.sy, (dd).g.sz})\n";
@@ -1030,7 +1077,7 @@ int32_t HPMCgetCellCodes(struct HPMCHistoPyramid* h, uint8_t* codes)
     if (h->broken) return fail(h, HPMC_ERR_BROKEN, "h is broken");
     if (h->tainted && !setup(h)) return h->last_error;
     if (h->mode == kFieldVolume) {
-        if (hpmc_host::plan_set_field(h->plan, 0, h->volume, h->stream) != hipSuccess)
+        if (hpmc_host::plan_set_field(h->plan, 0, volume_ptr(h), h->stream) != hipSuccess)
             return fail(h, HPMC_ERR_HIP, "descriptor update failed");
     }
     const hpmc::Geom& g = h->plan.vols[0].g;
@@ -1126,7 +1173,7 @@ int32_t HPMCadjoint(struct HPMCTraversalHandle* th, const float* grad_vertices, 
     HPMCHistoPyramid* h = th->h;
     if (h->broken) return fail(h, HPMC_ERR_BROKEN, "HistoPyramid is broken");
     if (h->tainted || !h->built) return fail(h, HPMC_ERR_ARG, "HistoPyramid has not been built since it was configured");
-    if (h->mode != kFieldVolume || h->gradient || h->binary_active)
+    if (h->mode != kFieldVolume || h->gradient || h->unorm8 || h->binary_active)
         return fail(h, HPMC_ERR_UNSUPPORTED, "the adjoint covers continuous FP32 volume fields");
     if (!wait_count(h)) return h->last_error;
     if (h->count && !grad_vertices) return fail(h, HPMC_ERR_NULL, "grad_vertices == NULL");

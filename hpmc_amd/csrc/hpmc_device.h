@@ -272,6 +272,55 @@ struct VolumeField {
     {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p, col, row, 0));
     }
+    // the sample value emission interpolates (fetch returns it as is)
+    __device__ __forceinline__ float sample(Plane p, uint32_t idx) const { return fetch(p, idx); }
+};
+
+// 8-bit normalised volume, the texture3d demo's field (a GL_ALPHA8 texture
+// sampled NEAREST with clamping, texture3d.cpp:127-141): sample value
+// c / 255, the GL unsigned-normalised conversion, correctly rounded.
+// Classification reads raw bytes (fetch, fetch2 return c as a float) and is
+// launched with the integer threshold kT = min{c : c / 255 >= T}, so c < kT
+// is the same predicate as c / 255 < T; emission stages the converted
+// values (sample) and runs unchanged.
+struct Volume8Field {
+    const uint8_t* __restrict__ v;
+    int sx, sy, sz;
+    __device__ __forceinline__ uint32_t index(int i, int j, int k) const
+    {
+        i = min(max(i, 0), sx - 1);
+        j = min(max(j, 0), sy - 1);
+        k = min(max(k, 0), sz - 1);
+        return ((uint32_t)k * (uint32_t)sy + (uint32_t)j) * (uint32_t)sx + (uint32_t)i;
+    }
+    __device__ __forceinline__ float at(int i, int j, int k) const { return (float)v[index(i, j, k)] / 255.0f; }
+    typedef __amdgpu_buffer_rsrc_t Plane;
+    __device__ __forceinline__ uint32_t lane_index(int i, int j) const
+    {
+        i = min(max(i, 0), sx - 1);
+        j = min(max(j, 0), sy - 1);
+        return (uint32_t)j * (uint32_t)sx + (uint32_t)i;
+    }
+    __device__ __forceinline__ Plane plane(int k) const
+    {
+        k = min(max(k, 0), sz - 1);
+        const uint8_t* base = v + (size_t)k * sy * sx;
+        return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, sx * sy, 0x00020000);
+    }
+    __device__ __forceinline__ float fetch(Plane p, uint32_t idx) const
+    {
+        return (float)__builtin_amdgcn_raw_buffer_load_b8(p, idx, 0, 0);
+    }
+    __device__ __forceinline__ uint32_t col_index(int i) const { return (uint32_t)min(max(i, 0), sx - 1); }
+    __device__ __forceinline__ uint32_t row_index(int j) const
+    {
+        return (uint32_t)min(max(j, 0), sy - 1) * (uint32_t)sx;
+    }
+    __device__ __forceinline__ float fetch2(Plane p, uint32_t col, uint32_t row) const
+    {
+        return (float)__builtin_amdgcn_raw_buffer_load_b8(p, col, row, 0);
+    }
+    __device__ __forceinline__ float sample(Plane p, uint32_t idx) const { return fetch(p, idx) / 255.0f; }
 };
 
 // RGBA FP32 volume carrying the gradient in rgb and the scalar in alpha, the
@@ -307,6 +356,7 @@ struct Volume4Field {
     {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p, idx, 0, 0));
     }
+    __device__ __forceinline__ float sample(Plane p, uint32_t idx) const { return fetch(p, idx); }
     __device__ __forceinline__ uint32_t col_index(int i) const { return (uint32_t)min(max(i, 0), sx - 1) * 16u + 12u; }
     __device__ __forceinline__ uint32_t row_index(int j) const
     {
@@ -907,7 +957,7 @@ __device__ __forceinline__ void emit_fetch(const F& field, int x0, int y0, int z
         if (q < n) {
             const typename F::Plane pl = field.plane(zfirst + q);
 #pragma unroll
-            for (int k = 0; k < kPlaneLoads; ++k) r[q][k] = field.fetch(pl, idx[k]);
+            for (int k = 0; k < kPlaneLoads; ++k) r[q][k] = field.sample(pl, idx[k]);
         }
     }
 }

@@ -71,6 +71,15 @@ void HPMCsetGridExtent(struct HPMCHistoPyramid* h, float x_extent, float y_exten
  * the upper faces) clamp to the edge sample, as a NEAREST/CLAMP texture. */
 void HPMCsetFieldTexture3D(struct HPMCHistoPyramid* h, const float* volume, uint8_t gradient);
 
+/* An 8-bit normalised volume, the field of the reference's texture3d demo
+ * (a GL_ALPHA8 texture sampled NEAREST with clamping,
+ * apps/texture3d/texture3d.cpp:127-141): `volume` is an x-fastest device
+ * array of the lattice size whose byte c samples as c / 255 (correctly
+ * rounded).  Setter semantics as HPMCsetFieldTexture3D (hpiface.cpp:164-184);
+ * no gradient variant, no adjoint.  No counterpart function in the
+ * reference, where the texture's format decides. */
+void HPMCsetFieldTexture3DUnorm8(struct HPMCHistoPyramid* h, const uint8_t* volume);
+
 /* Replaces HPMCsetFieldCustom (hpmc.h:439-443, hpiface.cpp:187-200).
  * `source` is HIP C++ defining
  *     __device__ float HPMC_fetch(float3 p);

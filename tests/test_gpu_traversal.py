@@ -3,8 +3,8 @@ HPMCgetTraversalView, HPMCgetTraversalGrid; the reference's
 triface.cpp:125-192 traversal functions for the caller's own shader).  A
 kernel compiled with hipRTC from the returned source visits the build's
 vertices through hpmc_for_each_vertex.  The bar: every vertex exactly once,
-with the extraction's index and bits, for volume, snippet, binary and
-gradient fields, on the suggested grid and on one that is not a multiple of
+with the extraction's index and bits, for volume, snippet, binary, gradient
+and 8-bit fields, on the suggested grid and on one that is not a multiple of
 eight, and again after a rebuild at another iso value."""
 import ctypes
 
@@ -117,6 +117,11 @@ def _case(kind):
     elif kind == "snippet":
         iso.set_snippet(workloads.CAYLEY_SNIPPET)
         T = 0.0
+    elif kind == "unorm8":
+        b = np.clip(np.rint(smooth_noise(lattice, 8) * 255.0), 0, 255).astype(np.uint8)
+        keep = torch.from_numpy(b).cuda()
+        iso.set_volume(keep)
+        T = 0.5
     elif kind == "binary":
         iso.set_binary(True)
         keep = torch.from_numpy((smooth_noise(lattice, 6) > 0.5).astype(np.float32)).cuda()
@@ -129,7 +134,7 @@ def _case(kind):
     return iso, keep, T
 
 
-@pytest.mark.parametrize("kind", ["volume", "snippet", "binary", "gradient"])
+@pytest.mark.parametrize("kind", ["volume", "snippet", "binary", "gradient", "unorm8"])
 def test_user_kernel_visits_the_extracted_vertices(hip, kind):
     iso, keep, T = _case(kind)
     ref, _ = iso.extract_all(T)
