@@ -226,6 +226,48 @@ def measure_pipelined(n, steps, warmup, inflight=3):
                     "steady-state interval, not the latency of one frame"}
 
 
+def measure_unorm8(n, steps, warmup):
+    """The Cayley field quantised to an 8-bit normalised volume (byte
+    clamp(round(128 + 64 f)), iso 128/255: the same surface to within the
+    quantisation), build + extract on one stream; 1 byte per lattice point."""
+    import torch
+    import hpmc_amd
+    dev = torch.device("cuda", 0)
+    f = cayley_volume_torch(n, dev)
+    vol = torch.clamp(torch.round(128.0 + 64.0 * f), 0, 255).to(torch.uint8).contiguous()
+    del f
+    T = 128.0 / 255.0
+    iso = hpmc_amd.IsoSurface((n, n, n))
+    iso.set_volume(vol)
+    iso.build(T)
+    nv = iso.count()
+    out = torch.empty((max(nv, 1), 6), dtype=torch.float32, device=dev)
+    iso.bind_output(out)
+    for _ in range(warmup):
+        iso.build(T)
+        iso.extract()
+    torch.cuda.synchronize()
+    iso.stage_times()
+    iso.set_profiling(True)
+    for _ in range(steps):
+        iso.build(T)
+        iso.extract()
+    torch.cuda.synchronize()
+    iso.set_profiling(False)
+    st = iso.stage_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        iso.build(T)
+        iso.extract()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    iso.close()
+    stage_ms = {k: v[0] / max(v[1], 1) for k, v in st.items()}
+    return {"workload": "cayley-%d^3 quantised to uint8 (128 + 64 f), iso 128/255" % n, "vertices": nv,
+            "ms_per_step": ms, "cells_per_s": (n - 1) ** 3 / (ms * 1e-3), "stage_ms": stage_ms,
+            "classify_GBps": n ** 3 / (stage_ms["classify"] * 1e-3) / 1e9}
+
+
 def traffic():
     """HBM bytes per classify launch from the committed counter profile of this
     workload, or None when the profile is absent."""
@@ -248,6 +290,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched/plain comparison")
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--no-pipeline", action="store_true", help="skip the frames-in-flight measurement")
+    ap.add_argument("--no-unorm8", action="store_true", help="skip the 8-bit volume measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 512 MiB copy instead (counter calibration)")
     args = ap.parse_args()
@@ -338,6 +381,8 @@ def main():
         pl = measure_pipelined(n, args.steps, args.warmup)
         pl["hbm_frac"] = algo_bytes / (pl["ms_per_frame"] * 1e-3) / (HBM_PEAK_GBS * 1e9)
         line["pipelined"] = pl
+    if not args.no_unorm8:
+        line["unorm8"] = measure_unorm8(n, args.steps, args.warmup)
     if not args.no_batch:
         line["batched"] = measure_batched(args.batch_reps)
     if not args.no_cpu_baseline:
