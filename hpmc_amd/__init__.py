@@ -137,6 +137,22 @@ def check_snippet(source, gradient=False, binary=False):
     return buf.value.decode()
 
 
+# Output buffers kept across extractions start at the reference demo's 3000
+# vertices and grow to 1.1 x the count when it no longer fits
+# (apps/transform_feedback/transform_feedback.cpp:341-349, 393-402).
+INITIAL_CAPACITY = 3 * 1000
+
+
+def grown_capacity(capacity, n):
+    """The capacity a kept output buffer needs for n vertices: unchanged
+    while it holds them, else (GLsizei)(1.1f * n) as the reference's demo
+    computes it (float32 product, truncated), never below n."""
+    import numpy as np
+    if capacity >= n:
+        return capacity
+    return max(int(np.float32(1.1) * np.float32(n)), n)
+
+
 def _libc_free(p):
     """free() for strings the library hands out with malloc."""
     libc = ctypes.CDLL(None)
@@ -180,6 +196,7 @@ class IsoSurface:
         self.th = None
         self._volume = None
         self._out = None
+        self._kept = None            # (vertices, cell_edge or None) of extract_growing
         self._ce = None
 
     # -- field ---------------------------------------------------------------
@@ -297,6 +314,28 @@ class IsoSurface:
         if n:
             self.extract()
         return out[:n], (ce[:n] if ce is not None else None)
+
+    def extract_growing(self, iso, with_cell_edge=False):
+        """Build, count and extract into buffers kept across calls, grown
+        like the reference's transform-feedback demo (grown_capacity):
+        returns views of the first N rows, valid until the next call."""
+        import torch
+        self.build(iso)
+        n = self.count()
+        kept = self._kept
+        cap = kept[0].shape[0] if kept is not None else 0
+        has_ce = kept is not None and kept[1] is not None
+        new_cap = grown_capacity(max(cap, INITIAL_CAPACITY) if kept is None else cap, n)
+        if kept is None or new_cap != cap or (with_cell_edge and not has_ce):
+            dev = torch.device("cuda", torch.cuda.current_device())
+            out = torch.empty((new_cap, 6), dtype=torch.float32, device=dev)
+            ce = torch.empty((new_cap, 2), dtype=torch.int32, device=dev) if with_cell_edge else None
+            self._kept = (out, ce)
+        out, ce = self._kept
+        self.bind_output(out, ce if with_cell_edge else None)
+        if n:
+            self.extract()
+        return out[:n], (ce[:n] if with_cell_edge else None)
 
     def cell_codes(self):
         import torch

@@ -95,3 +95,21 @@ def test_traversal_source_is_valid_hip():
     # the device source HPMCgetTraversalShaderFunctions hands out compiles
     src = open(os.path.join(ROOT, "hpmc_amd", "csrc", "hpmc_device.h")).read()
     assert "emit_body" in src and "classify_body" in src
+
+
+def test_output_buffer_growth_policy():
+    # apps/transform_feedback/transform_feedback.cpp:393-402: keep the buffer
+    # while it holds N, else (GLsizei)(1.1f * N)
+    from hpmc_amd import INITIAL_CAPACITY, grown_capacity
+    assert INITIAL_CAPACITY == 3000
+    assert grown_capacity(3000, 2999) == 3000
+    assert grown_capacity(3000, 3000) == 3000
+    assert grown_capacity(3000, 3001) == 3301          # 1.1f * 3001 = 3301.1
+    assert grown_capacity(3301, 3301) == 3301
+    assert grown_capacity(3301, 10000) == 11000
+    # float32 product, truncated: 1.1f is 1.10000002384..., so 1.1f * 3694104
+    # = 4063514.49 -> 4063514
+    assert grown_capacity(0, 3694104) == 4063514
+    for n in (1, 7, 12345, 2 ** 24 + 1, 3 * 10 ** 9):
+        c = grown_capacity(0, n)
+        assert n <= c <= int(1.1 * n) + 1

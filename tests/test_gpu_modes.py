@@ -105,3 +105,31 @@ def test_launch_shapes_do_not_change_results(tuning):
     assert torch.equal(ref.cell_codes(), iso.cell_codes())
     ref.close()
     iso.close()
+
+
+def test_kept_output_buffer_grows_like_the_reference_demo():
+    # apps/transform_feedback/transform_feedback.cpp:341-349, 393-402: one
+    # buffer across frames, grown to 1.1 x N only when N no longer fits
+    from hpmc_amd import grown_capacity
+    lattice = (80, 70, 60)
+    f = smooth_noise(lattice, 23)
+    vol = torch.from_numpy(f).cuda()
+    iso = IsoSurface(lattice)
+    iso.set_volume(vol)
+    ref = IsoSurface(lattice)
+    ref.set_volume(vol)
+    cap, ptr = 3000, None
+    for T in (0.95, 0.5, 0.9, 0.45, 0.5):
+        v, ce = iso.extract_growing(T, with_cell_edge=True)
+        rv, rce = ref.extract_all(T, with_cell_edge=True)
+        assert torch.equal(v, rv) and torch.equal(ce, rce)
+        n = rv.shape[0]
+        grew = n > cap
+        cap = grown_capacity(cap, n)
+        kept = iso._kept[0]
+        assert kept.shape[0] == cap
+        if ptr is not None:
+            assert (kept.data_ptr() != ptr) == grew
+        ptr = kept.data_ptr()
+    iso.close()
+    ref.close()
