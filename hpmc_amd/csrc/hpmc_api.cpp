@@ -520,6 +520,31 @@ const hpmc_host::DeviceShape& device_shape()
     return shape;
 }
 
+// Uploads the case and binary-normal tables once; false without a device.
+bool upload_tables(HPMCConstants* c)
+{
+    if (c->d_cases) return true;
+    static float4 bnorm[256 * 16];
+    static std::once_flag bnorm_once;
+    std::call_once(bnorm_once, [] { binary_normals(bnorm); });
+    uint64_t* cases = nullptr;
+    float4* bn = nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipMalloc(&cases, sizeof kCases) != hipSuccess ||
+        hipMemcpy(cases, kCases, sizeof kCases, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&bn, sizeof bnorm) != hipSuccess ||
+        hipMemcpy(bn, bnorm, sizeof bnorm, hipMemcpyHostToDevice) != hipSuccess) {
+        if (cases) (void)hipFree(cases);
+        if (bn) (void)hipFree(bn);
+        (void)hipGetLastError();
+        return false;
+    }
+    c->device = dev;
+    c->d_cases = cases;
+    c->d_bnorm = bn;
+    return true;
+}
+
 // hpmc/src/init.cpp:48-69: layout, storage and builder programs, once per
 // configuration.  Classification workgroups span whole lattice rows
 // (ceil(X/64) waves, at most 16) over zseg slices, zseg chosen so the
@@ -527,6 +552,10 @@ const hpmc_host::DeviceShape& device_shape()
 bool setup(HPMCHistoPyramid* h)
 {
     if (!h->tainted) return true;
+    if (!upload_tables(h->constants)) {
+        fail(h, HPMC_ERR_HIP, "no HIP device for the marching-cubes tables");
+        return false;
+    }
     free_device(h);
     if (!determine_layout(h)) return false;
     if (h->mode == kFieldVolume && !volume_ptr(h)) {
@@ -677,19 +706,10 @@ struct HPMCConstants* HPMCcreateConstants(int32_t, int32_t)
 {
     HPMCConstants* c = new (std::nothrow) HPMCConstants;
     if (!c) return nullptr;
-    static float4 bnorm[256 * 16];
-    static std::once_flag bnorm_once;
-    std::call_once(bnorm_once, [] { binary_normals(bnorm); });
-    if (hipGetDevice(&c->device) != hipSuccess || hipMalloc(&c->d_cases, sizeof kCases) != hipSuccess ||
-        hipMemcpy(c->d_cases, kCases, sizeof kCases, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMalloc(&c->d_bnorm, sizeof bnorm) != hipSuccess ||
-        hipMemcpy(c->d_bnorm, bnorm, sizeof bnorm, hipMemcpyHostToDevice) != hipSuccess) {
-        fail(nullptr, HPMC_ERR_HIP, "failed to upload marching-cubes tables");
-        if (c->d_cases) (void)hipFree(c->d_cases);
-        if (c->d_bnorm) (void)hipFree(c->d_bnorm);
-        delete c;
-        return nullptr;
-    }
+    // The tables go to the current HIP device now; without a usable device
+    // the handle is still made (setters, layout and traversal source work on
+    // the host) and the first build uploads them, failing there if it can't.
+    (void)upload_tables(c);
     ok(nullptr);
     return c;
 }
@@ -1244,6 +1264,7 @@ int32_t batch_setup(HPMCBatch* b)
         b->own_stream = true;
     }
     if (!b->tainted) return HPMC_OK;
+    if (!upload_tables(b->constants)) return fail(nullptr, HPMC_ERR_HIP, "no HIP device for the marching-cubes tables");
     batch_drop_graph(b);
     std::string err;
     const int32_t rc = hpmc_host::plan_create(b->plan, b->specs, b->tuning, (const void*)hpmc_classify_volume,

@@ -10,6 +10,7 @@ import subprocess
 import pytest
 
 import hpmc_amd
+import oracle
 from hpmc_amd import workloads
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -113,3 +114,61 @@ def test_output_buffer_growth_policy():
     for n in (1, 7, 12345, 2 ** 24 + 1, 3 * 10 ** 9):
         c = grown_capacity(0, n)
         assert n <= c <= int(1.1 * n) + 1
+
+
+def _histopyramid(lattice, grid=None):
+    L = hpmc_amd.lib()
+    c = L.HPMCcreateConstants(4, 3)
+    assert c, "constants are made without a device as well"
+    h = L.HPMCcreateHistoPyramid(c)
+    assert h
+    L.HPMCsetLatticeSize(h, *lattice)
+    if grid is not None:
+        L.HPMCsetGridSize(h, *grid)
+    return L, c, h
+
+
+@pytest.mark.parametrize("lattice", [(64, 64, 64), (256, 256, 256), (512, 512, 512), (33, 20, 45),
+                                     (150, 70, 90), (17, 300, 5), (1000, 10, 200), (16384, 16, 3)])
+def test_layout_query_matches_oracle(lattice):
+    # init.cpp:105-130 in FP32, host only
+    L, c, h = _histopyramid(lattice)
+    out = (ctypes.c_int32 * 5)()
+    assert L.HPMCgetLayout(h, out) == 0
+    cells = tuple(v - 1 for v in lattice)
+    assert tuple(out) == oracle.layout(lattice, cells), (tuple(out), lattice)
+    L.HPMCdestroyHandle(h)
+    L.HPMCdestroyConstants(c)
+
+
+@pytest.mark.parametrize("lattice,grid", [((1, 8, 8), None), ((8, 8, 16385), None), ((8, 8, 8), (8, 7, 7)),
+                                          ((8, 8, 8), (7, 0, 7)), ((8, 2, 8), None), ((6, 6, 6), None)])
+def test_layout_rejects_bad_sizes(lattice, grid):
+    # init.cpp:82-103: lattice in [2, 16384], 1 <= grid < lattice; this build
+    # also needs more than 8 cells along x and y (16-cell blocks of the
+    # doubled tiles) and at least 2 (the reference's log2f(0.5) shift)
+    L, c, h = _histopyramid(lattice, grid)
+    out = (ctypes.c_int32 * 5)()
+    code = L.HPMCgetLayout(h, out)
+    assert code in (3, 9), code           # HPMC_ERR_SIZE / HPMC_ERR_UNSUPPORTED
+    assert L.HPMCgetLastError(h) == code
+    L.HPMCdestroyHandle(h)
+    L.HPMCdestroyConstants(c)
+
+
+def test_build_without_device_fails_loudly():
+    # no CPU fallback: the build reports a HIP error and latches broken
+    # (hpiface.cpp:258-290), the count is 0, no traversal handle is made.
+    # (Asked inside the test, through torch: nothing may start a HIP runtime
+    # while the tests are collected.)
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("checks the no-device path")
+    L, c, h = _histopyramid((64, 64, 64))
+    L.HPMCsetFieldCustom(h, workloads.CAYLEY_SNIPPET.encode(), 0, 0)
+    L.HPMCbuildHistopyramid(h, 0.0)
+    assert L.HPMCgetLastError(h) == 6           # HPMC_ERR_HIP
+    assert L.HPMCacquireNumberOfVertices(h) == 0
+    assert not L.HPMCcreateTraversalHandle(h)
+    L.HPMCdestroyHandle(h)
+    L.HPMCdestroyConstants(c)
