@@ -45,7 +45,7 @@ HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume8, HPMC_VOLUME8_FIELD, hpmc::kEmitContin
 HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume8_binary, HPMC_VOLUME8_FIELD, hpmc::kEmitBinary, false)
 HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume_batch, HPMC_VOLUME_FIELD, hpmc::kEmitContinuous, true)
 
-extern "C" __global__ __launch_bounds__(256) void hpmc_scan(
+extern "C" __global__ __launch_bounds__(hpmc::kScanThreads) void hpmc_scan(
     const hpmc::VolDesc* vols, const uint32_t* tile_vol, uint32_t n_l4t, uint32_t n_colt, const uint16_t* colcnt,
     uint2* tile_sums, hpmc::ScanState* state, uint32_t* l4_off, uint2* items, hpmc::Totals* totals,
     hpmc::Totals* host_totals, hpmc::VolCount* vol_counts, hpmc::VolCount* host_vol_counts)

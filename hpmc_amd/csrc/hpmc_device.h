@@ -72,7 +72,8 @@ constexpr int kColStride = 32;
 
 // Scan tiles: 4096 entries, 16 per thread of a 256-thread workgroup.
 constexpr int kScanTile = 4096;
-constexpr int kScanPer = kScanTile / 256;   // entries per thread
+constexpr int kScanThreads = 256;                // threads per scan workgroup
+constexpr int kScanPer = kScanTile / kScanThreads;   // entries per thread
 
 // Emit work items hold at most kEmitChunk vertices of one block, so a busy
 // block is spread over several waves.
@@ -674,22 +675,19 @@ __device__ __forceinline__ void scan_load16(const uint32_t* __restrict__ level4,
 }
 
 
-// The 16 column-major block counts of thread t of a tile (u16, two 16-byte
-// loads); entries past n read as zero.
+// The kScanPer column-major block counts of thread t of a tile (u16, read
+// as 32-bit words that merge into vector loads); entries past n read as zero.
 __device__ __forceinline__ void scan_load16_u16(const uint16_t* __restrict__ cc, uint32_t n, uint32_t base,
                                                 uint32_t (&c)[kScanPer])
 {
+    static_assert(kScanPer % 2 == 0, "pairs of u16 per word");
     if (base + kScanPer <= n) {
-        const uint4* p = (const uint4*)(cc + base);
+        const uint32_t* p = (const uint32_t*)(cc + base);
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const uint4 q = p[k];
-            const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                c[8 * k + 2 * i] = w[i] & 0xffffu;
-                c[8 * k + 2 * i + 1] = w[i] >> 16;
-            }
+        for (int i = 0; i < kScanPer / 2; ++i) {
+            const uint32_t w = p[i];
+            c[2 * i] = w & 0xffffu;
+            c[2 * i + 1] = w >> 16;
         }
     } else {
 #pragma unroll

@@ -386,7 +386,7 @@ inline hipError_t plan_scan(const Plan& p, const void* k_scan, hipStream_t strea
     hpmc::VolCount* dv = p.d_vcounts;
     hpmc::VolCount* hv = p.h_vcounts;
     void* args[] = {&vols, &tv, &nl, &nc, &cc, &ts, &st, &lo, &it, &dt, &ht, &dv, &hv};
-    return hipLaunchKernel(k_scan, dim3(grid), dim3(256), args, 0, stream);
+    return hipLaunchKernel(k_scan, dim3(grid), dim3(hpmc::kScanThreads), args, 0, stream);
 }
 
 inline hipError_t plan_emit(const Plan& p, const Kernel& k, float T, float* out, hpmc::CellEdge* out_ce,
