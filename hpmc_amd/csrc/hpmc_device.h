@@ -653,28 +653,6 @@ __device__ __forceinline__ uint2 block_reduce_u2(uint2 v, uint2* sh)
     return r;
 }
 
-// The 16 entries of thread t of a tile, loaded as four 16-byte vectors
-// (all four in flight at once); entries past n read as zero.
-__device__ __forceinline__ void scan_load16(const uint32_t* __restrict__ level4, uint32_t n, uint32_t base,
-                                            uint32_t (&c)[kScanPer])
-{
-    if (base + kScanPer <= n) {
-        const uint4* p = (const uint4*)(level4 + base);
-#pragma unroll
-        for (int k = 0; k < kScanPer / 4; ++k) {
-            const uint4 q = p[k];
-            c[4 * k + 0] = q.x;
-            c[4 * k + 1] = q.y;
-            c[4 * k + 2] = q.z;
-            c[4 * k + 3] = q.w;
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < kScanPer; ++k) c[k] = (base + k < n) ? level4[base + k] : 0u;
-    }
-}
-
-
 // The kScanPer column-major block counts of thread t of a tile (u16, read
 // as 32-bit words that merge into vector loads); entries past n read as zero.
 __device__ __forceinline__ void scan_load16_u16(const uint16_t* __restrict__ cc, uint32_t n, uint32_t base,
