@@ -502,7 +502,13 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
     w.xin = w.x < g.sx;
     w.xcell = w.x < g.cx;
     w.has_right = w.x0 + 64 < g.sx;
-    const bool last = wv == nw - 1;
+    // Waves wholly right of the lattice (narrow volumes of a batch under a
+    // wider volume's workgroup shape) leave at once: the barriers below wait
+    // only for the waves still running, and the last wave with columns takes
+    // the right-neighbour role.
+    if (w.x0 >= g.sx) return;
+    const int nw_active = min(nw, (g.sx - bx * nw * 64 + 63) >> 6);
+    const bool last = wv == nw_active - 1;
     // right column from the next workgroup (wave-uniform: a scalar branch)
     const bool load_e = __builtin_amdgcn_readfirstlane((int)(last && w.has_right)) != 0;
     w.lane_e = load_e && lane <= kBlock && (w.y0 + lane < g.sy);
@@ -618,7 +624,7 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
     __syncthreads();
     if (wv == 0 && lane < ns) {
         uint2 r = make_uint2(0, 0);
-        for (int i = 0; i < nw; ++i) {
+        for (int i = 0; i < nw_active; ++i) {
             r.x += red[i][lane].x;
             r.y += red[i][lane].y;
         }
