@@ -1492,10 +1492,12 @@ int32_t HPMCbatchAutotune(struct HPMCBatch* b, float threshold, float* out, uint
     if (!b || !out) return fail(nullptr, HPMC_ERR_NULL, "NULL argument");
     if (b->specs.empty()) return fail(nullptr, HPMC_ERR_ARG, "empty batch");
     if (repetitions < 1) repetitions = 1;
+    // classification waves: powers of two up to 8 (narrow volumes stack row
+    // bands, so more waves than a row needs still pay) and whole rows of the
+    // widest volume
     int max_row_waves = 1;
     for (const auto& v : b->specs) max_row_waves = std::max(max_row_waves, hpmc_host::row_waves(v.size[0]));
-    std::vector<int> waves = {1};
-    for (int w = 2; w <= max_row_waves; w *= 2) waves.push_back(w);
+    std::vector<int> waves = {1, 2, 4, 8};
     if (std::find(waves.begin(), waves.end(), max_row_waves) == waves.end()) waves.push_back(max_row_waves);
     const hpmc_host::DeviceShape& dev = device_shape();
     const int full = (dev.cus > 0 ? dev.cus : 256) * (dev.emit_per_cu > 0 ? dev.emit_per_cu : 4);

@@ -464,13 +464,22 @@ __device__ __forceinline__ uint32_t classify_slice_counts(const ClassifyWave& w,
 // to logical indices so that each XCD runs a contiguous run (classify
 // kernel wrapper): workgroups resident together on one XCD are y-neighbours
 // and share their halo rows in that XCD's L2.
-__device__ __forceinline__ void classify_tile(const Geom& g, int zseg, int wgx, int t, int& bx, int& py, int& pz)
+// A workgroup of nw waves spans wx = min(nw, ceil(X / 64)) waves across x
+// and stacks nw / wx row bands, so narrow lattices use every wave.
+__device__ __forceinline__ void classify_shape(const Geom& g, int nw, int& wx, int& nstack)
+{
+    wx = min(nw, (g.sx + 63) >> 6);
+    nstack = nw / wx;
+}
+
+__device__ __forceinline__ void classify_tile(const Geom& g, int nstack, int wgx, int t, int& bx, int& pyg, int& pz)
 {
     const int npy = (g.cy + kBlock - 1) >> kBlockLog2;
+    const int npyg = (npy + nstack - 1) / nstack;
     bx = t % wgx;
     const int rest = t / wgx;
-    py = rest % npy;
-    pz = rest / npy;
+    pyg = rest % npyg;
+    pz = rest / npyg;
 }
 
 template <class F>
@@ -485,16 +494,19 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nw = blockDim.x >> 6;
-    const int wgx = (g.sx + 64 * nw - 1) / (64 * nw);    // workgroups across x
-    int bx, py, pz;
-    classify_tile(g, zseg, wgx, t, bx, py, pz);
+    int wx, nstack;
+    classify_shape(g, nw, wx, nstack);
+    const int wgx = (g.sx + 64 * wx - 1) / (64 * wx);    // workgroups across x
+    int bx, pyg, pz;
+    classify_tile(g, nstack, wgx, t, bx, pyg, pz);
     bx = __builtin_amdgcn_readfirstlane(bx);
-    py = __builtin_amdgcn_readfirstlane(py);
+    pyg = __builtin_amdgcn_readfirstlane(pyg);
     pz = __builtin_amdgcn_readfirstlane(pz);
+    const int xw = wv % wx, yw = wv / wx;   // this wave's place: column group, stacked band
     ClassifyWave w;
     w.lane = lane;
-    w.x0 = (bx * nw + wv) << 6;
-    w.y0 = py << kBlockLog2;
+    w.x0 = (bx * wx + xw) << 6;
+    w.y0 = (pyg * nstack + yw) << kBlockLog2;
     w.x = w.x0 + lane;
     w.sy = g.sy;
     w.cx = g.cx;
@@ -502,13 +514,12 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
     w.xin = w.x < g.sx;
     w.xcell = w.x < g.cx;
     w.has_right = w.x0 + 64 < g.sx;
-    // Waves wholly right of the lattice (narrow volumes of a batch under a
-    // wider volume's workgroup shape) leave at once: the barriers below wait
-    // only for the waves still running, and the last wave with columns takes
-    // the right-neighbour role.
-    if (w.x0 >= g.sx) return;
-    const int nw_active = min(nw, (g.sx - bx * nw * 64 + 63) >> 6);
-    const bool last = wv == nw_active - 1;
+    // Waves with no lattice columns or no cell rows (past the stacked bands,
+    // past the lattice) leave at once: the barriers below wait only for the
+    // waves still running.  The last wave of a band's row of waves takes the
+    // right-neighbour role.
+    if (yw >= nstack || w.x0 >= g.sx || w.y0 >= g.cy) return;
+    const bool last = xw == wx - 1;
     // right column from the next workgroup (wave-uniform: a scalar branch)
     const bool load_e = __builtin_amdgcn_readfirstlane((int)(last && w.has_right)) != 0;
     w.lane_e = load_e && lane <= kBlock && (w.y0 + lane < g.sy);
@@ -574,7 +585,7 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
     // lane l writes entries 2(l%16), 2(l%16)+1 of block column l/16
     wave_lds_sync();
     const int bcol = (w.x0 >> kBlockLog2) + (lane >> 4);
-    const uint32_t wi = ((uint32_t)pz * g.nby + py) * g.nbx + bcol;
+    const uint32_t wi = ((uint32_t)pz * g.nby + (uint32_t)(w.y0 >> kBlockLog2)) * g.nbx + bcol;
     uint32_t vsum = 0, isum = 0;
     if (bcol < g.nbx) {
         const uint32_t lo = mycnt[lane >> 4][(lane & 15) * 2], hi = mycnt[lane >> 4][(lane & 15) * 2 + 1];
@@ -624,7 +635,10 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
     __syncthreads();
     if (wv == 0 && lane < ns) {
         uint2 r = make_uint2(0, 0);
-        for (int i = 0; i < nw_active; ++i) {
+        for (int i = 0; i < nw; ++i) {
+            // waves that left wrote nothing
+            const int ix = i % wx, iy = i / wx;
+            if (iy >= nstack || ((bx * wx + ix) << 6) >= g.sx || ((pyg * nstack + iy) << kBlockLog2) >= g.cy) continue;
             r.x += red[i][lane].x;
             r.y += red[i][lane].y;
         }

@@ -91,10 +91,16 @@ inline int row_waves(int sx)
     return nw > hpmc::kClassifyMaxWaves ? hpmc::kClassifyMaxWaves : nw;
 }
 
+// Classification workgroups of one slice segment: nw waves span
+// wx = min(nw, ceil(X/64)) waves across x and stack nw / wx row bands
+// (classify_shape in the kernels).
 inline int64_t classify_bands(const hpmc::Geom& g, int nw)
 {
-    const int64_t wgx = (g.sx + 64 * nw - 1) / (64 * nw);
-    return wgx * ((g.cy + hpmc::kBlock - 1) / hpmc::kBlock);
+    const int wx = std::min(nw, (g.sx + 63) / 64);
+    const int sb = nw / wx;
+    const int64_t wgx = (g.sx + 64 * wx - 1) / (64 * wx);
+    const int64_t npy = (g.cy + hpmc::kBlock - 1) / hpmc::kBlock;
+    return wgx * ((npy + sb - 1) / sb);
 }
 
 // zseg so the workgroups of `cells_z` slices over `bands` row bands fill
@@ -235,9 +241,10 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
         }
         max_row_waves = std::max(max_row_waves, row_waves(specs[i].size[0]));
     }
-    // one workgroup shape per launch: whole rows of the widest volume unless tuned
+    // one workgroup shape per launch unless tuned: whole rows of the widest
+    // volume, at least 4 waves (narrower volumes stack row bands)
     p.cls_waves = tune.classify_waves > 0 ? std::min(tune.classify_waves, (int)hpmc::kClassifyMaxWaves)
-                                          : max_row_waves;
+                                          : std::max(max_row_waves, 4);
     int per_cu = 0;
     if (classify_kernel)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, classify_kernel, 64 * p.cls_waves, 0);
