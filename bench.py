@@ -163,6 +163,36 @@ def measure_batched(reps, count=256, iso=1.0):
     return res
 
 
+def batch_steps(steps, warmup, tuning=(8, 16, 1280), count=256, iso=1.0):
+    """The batched workload's build + extract under a fixed launch shape, for
+    kernel traces (bench.py --batch-trace): `warmup` then `steps` steps."""
+    import torch
+    import hpmc_amd
+    from hpmc_amd import workloads
+    dev = torch.device("cuda", 0)
+    specs = workloads.batch_volume_specs(count=count, seed=0)
+    vols = [workloads.metaballs_torch(size, balls, dev) for size, balls in specs]
+    b = hpmc_amd.Batch()
+    for v in vols:
+        b.add_volume(v)
+    b.set_tuning(*tuning)
+    b.build(iso)
+    _, _, total = b.counts()
+    out = torch.empty((total, 6), dtype=torch.float32, device=dev)
+    for _ in range(warmup):
+        b.build(iso)
+        b.extract(out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.build(iso)
+        b.extract(out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    b.close()
+    print(json.dumps({"batch_trace": {"tuning": list(tuning), "vertices": int(total), "ms_per_step": ms}}))
+
+
 def copy_calibration(steps):
     """A plain 512 MiB device copy (torch copy_), for scaling the HBM counters:
     each launch reads and writes exactly 536,870,912 bytes."""
@@ -291,6 +321,8 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--no-pipeline", action="store_true", help="skip the frames-in-flight measurement")
     ap.add_argument("--no-unorm8", action="store_true", help="skip the 8-bit volume measurement")
+    ap.add_argument("--batch-trace", action="store_true",
+                    help="only the batched build + extract at a fixed launch shape (for kernel traces)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 512 MiB copy instead (counter calibration)")
     args = ap.parse_args()
@@ -301,6 +333,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.steps)
+        return
+    if args.batch_trace:
+        batch_steps(args.steps, args.warmup)
         return
 
     import torch
