@@ -103,7 +103,8 @@ def test_batch_volumes_against_oracle(volumes):
     b.close()
 
 
-@pytest.mark.parametrize("waves,zseg,emit", [(1, 8, 8), (1, 32, 64), (2, 16, 0), (2, 8, 256), (0, 32, 24)])
+@pytest.mark.parametrize("waves,zseg,emit", [(1, 8, 8), (1, 32, 64), (2, 16, 0), (2, 8, 256), (0, 32, 24),
+                                            (3, 16, 0), (8, 16, 0), (16, 8, 40)])
 def test_batch_every_tuning_identical(volumes, plain, waves, zseg, emit):
     b = _new_batch(volumes)
     b.set_tuning(waves, zseg, emit)

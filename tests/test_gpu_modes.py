@@ -85,11 +85,13 @@ def test_gradient_snippet_cayley():
     assert check(res, f, 0.0, lattice, clamp=False, mode="gradient", field4=f4) > 10000
 
 
-@pytest.mark.parametrize("tuning", [(1, 8, 8), (4, 16, 256), (2, 32, 0)])
-def test_launch_shapes_do_not_change_results(tuning):
+@pytest.mark.parametrize("lattice,tuning", [((150, 70, 90), (1, 8, 8)), ((150, 70, 90), (4, 16, 256)),
+                                            ((150, 70, 90), (2, 32, 0)), ((40, 70, 90), (8, 16, 0)),
+                                            ((70, 33, 50), (16, 8, 0)), ((200, 20, 30), (5, 32, 16))])
+def test_launch_shapes_do_not_change_results(lattice, tuning):
     # HPMCsetTuning: launch shapes of one HistoPyramid; every shape gives the
-    # same codes, count, (cell, edge) list and vertex bits
-    lattice = (150, 70, 90)
+    # same codes, count, (cell, edge) list and vertex bits (narrow lattices
+    # stack row bands in a workgroup; waves past the rows or columns leave)
     f = smooth_noise(lattice, 17)
     vol = torch.from_numpy(f).cuda()
     ref = IsoSurface(lattice)
