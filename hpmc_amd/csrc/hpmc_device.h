@@ -983,7 +983,8 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                                           const uint32_t nitems,
                                           float* __restrict__ out, CellEdge* __restrict__ out_ce,
                                           uint32_t capacity, const float4* __restrict__ bnorm, EmitShared& sh,
-                                          const AdjointIO adj = AdjointIO{}, const CB& cb = CB{})
+                                          uint64_t case_word, const AdjointIO adj = AdjointIO{},
+                                          const CB& cb = CB{})
 {
     constexpr int NP = emit_planes<MODE>();
     const int lane = threadIdx.x & 63;
@@ -999,7 +1000,7 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     const uint64_t wlog = (uint64_t)t * kEmitWaves + wv;
     uint32_t it = __builtin_amdgcn_readfirstlane((uint32_t)(wlog * nitems / nwaves));
     const uint32_t iend = __builtin_amdgcn_readfirstlane((uint32_t)((wlog + 1) * nitems / nwaves));
-    if (it >= iend) return;
+    const bool has_items = it < iend;
 
     EmitWaveShared& ws = sh.w[wv];
     const uint64_t* cases = sh.cases;
@@ -1008,7 +1009,18 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     const int qx = (int)morton_compact((uint32_t)lane) * 2;
     const int qy = (int)morton_compact((uint32_t)lane >> 1) * 2;
 
-    uint2 cur = load_uniform(items + it);
+    // The first item's entry, descriptor, offset and planes are requested
+    // before this thread's case-table entry (case_word, loaded by the
+    // caller) is stored to LDS and the workgroup barrier publishes the
+    // tables, so the round trips overlap.
+    // (Loaded whether or not the wave has items, so that the count of loads
+    // in flight is fixed and the wait for case_word does not cover them; a
+    // wave without items reads item slot 0 and uses the key-0 block of
+    // volume 0, all in bounds.)
+    const uint32_t i0 = has_items ? it : 0u;
+    uint2 cur = load_uniform(items + i0);
+    uint2 nxt = load_uniform(items + (has_items ? min(it + 1, iend - 1) : 0u));
+    if (!has_items) cur = make_uint2(0u, 0u);
     VolDesc dn = load_uniform(vols + (BATCH ? item_volume(cur.y) : 0u));
     uint32_t obase = load_uniform(l4_off + dn.l4_base + cur.x);
     float r[3][kPlaneLoads];
@@ -1016,7 +1028,10 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     block_origin(dn.g, cur.x, x0, y0, z);
     int nnew = NP;                      // planes the item needs that are not staged yet
     emit_fetch(mk(dn), x0, y0, z, NP, lane, r);
-    uint2 nxt = load_uniform(items + min(it + 1, iend - 1));
+    sh.cases[threadIdx.x] = case_word;
+    sh.lut[threadIdx.x] = (uint8_t)(case_word & 0xfu);
+    __syncthreads();
+    if (!has_items) return;
 
     for (;;) {
         const uint32_t vlo = item_chunk_first(cur.y), cnt = item_block_count(cur.y);
@@ -1370,15 +1385,10 @@ template <int MODE, class MK, class CB>
 __device__ __forceinline__ void for_each_vertex(const TraversalView& v, const MK& mk, const CB& cb)
 {
     __shared__ EmitShared sh;
+    if (blockDim.x != 64 * kEmitWaves) return;   // the emission loop needs 4 waves (one table entry a thread)
     const uint32_t nitems = load_uniform(&v.totals->items);
-    for (int i = threadIdx.x; i < 256; i += blockDim.x) {
-        sh.cases[i] = v.cases[i];
-        sh.lut[i] = (uint8_t)(v.cases[i] & 0xfu);
-    }
-    __syncthreads();
-    if (blockDim.x != 64 * kEmitWaves) return;   // the emission loop needs 4 waves per workgroup
     emit_body<MODE, false>(v.vols, mk, v.threshold, v.items, v.l4_off, nitems, nullptr, nullptr, 0xffffffffu,
-                           v.bnorm, sh, AdjointIO{}, cb);
+                           v.bnorm, sh, v.cases[threadIdx.x], AdjointIO{}, cb);
 }
 
 }  // namespace hpmc
@@ -1423,12 +1433,10 @@ __device__ __forceinline__ void for_each_vertex(const TraversalView& v, const MK
     {                                                                                         \
         __shared__ hpmc::EmitShared sh;                                                       \
         const uint32_t nitems = hpmc::load_uniform(&totals->items);                           \
-        sh.cases[threadIdx.x] = cases[threadIdx.x];                                           \
-        sh.lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);                           \
-        __syncthreads();                                                                      \
+        const uint64_t case_word = cases[threadIdx.x];   /* stored to LDS by emit_body */     \
         auto mk = [](const hpmc::VolDesc& dd) { return MAKE_FIELD; };                         \
         hpmc::emit_body<MODE, BATCH>(vols, mk, T, items, l4_off, nitems, out, out_ce,         \
-                                     capacity, bnorm, sh);                                    \
+                                     capacity, bnorm, sh, case_word);                         \
     }
 
 // Adjoint of continuous emission (kEmitAdjoint) over the same items.
@@ -1440,11 +1448,9 @@ __device__ __forceinline__ void for_each_vertex(const TraversalView& v, const MK
     {                                                                                         \
         __shared__ hpmc::EmitShared sh;                                                       \
         const uint32_t nitems = hpmc::load_uniform(&totals->items);                           \
-        sh.cases[threadIdx.x] = cases[threadIdx.x];                                           \
-        sh.lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);                           \
-        __syncthreads();                                                                      \
+        const uint64_t case_word = cases[threadIdx.x];   /* stored to LDS by emit_body */     \
         auto mk = [](const hpmc::VolDesc& dd) { return MAKE_FIELD; };                         \
         hpmc::emit_body<hpmc::kEmitAdjoint, BATCH>(vols, mk, T, items, l4_off, nitems, nullptr, \
-                                                   nullptr, count, nullptr, sh,               \
+                                                   nullptr, count, nullptr, sh, case_word,    \
                                                    hpmc::AdjointIO{gout, keys, vals, dT});    \
     }
