@@ -153,6 +153,25 @@ def grown_capacity(capacity, n):
     return max(int(np.float32(1.1) * np.float32(n)), n)
 
 
+def load_raw_volume(path, size):
+    """Reads an 8-bit raw volume file (x-fastest bytes, no header) for a
+    lattice `size` = (x, y, z) into a uint8 array shaped (z, y, x), the
+    input of IsoSurface.set_volume's 8-bit path.  Like the reference's
+    texture demo (apps/texture3d/texture3d.cpp:331-344): the first x*y*z
+    bytes are used, a shorter file leaves the rest zero, and a file that
+    cannot be opened is an error."""
+    import numpy as np
+    x, y, z = (int(v) for v in size)
+    if min(x, y, z) < 1:
+        raise ValueError("lattice size must be positive: %r" % (size,))
+    n = x * y * z
+    with open(path, "rb") as f:
+        data = f.read(n)
+    vol = np.zeros(n, np.uint8)
+    vol[:len(data)] = np.frombuffer(data, np.uint8)
+    return vol.reshape(z, y, x)
+
+
 def _libc_free(p):
     """free() for strings the library hands out with malloc."""
     libc = ctypes.CDLL(None)

@@ -172,3 +172,27 @@ def test_build_without_device_fails_loudly():
     assert not L.HPMCcreateTraversalHandle(h)
     L.HPMCdestroyHandle(h)
     L.HPMCdestroyConstants(c)
+
+
+def test_raw_volume_loader(tmp_path):
+    # apps/texture3d/texture3d.cpp:331-344: x*y*z bytes, x fastest; a short
+    # file leaves the rest zero, a long one is cut, a missing one is an error
+    import numpy as np
+    from hpmc_amd import load_raw_volume
+    size = (5, 4, 3)
+    data = bytes((7 * i + 3) % 256 for i in range(60))
+    p = tmp_path / "v.raw"
+    p.write_bytes(data)
+    v = load_raw_volume(str(p), size)
+    assert v.dtype == np.uint8 and v.shape == (3, 4, 5)
+    assert v[2, 1, 4] == data[4 + 5 * (1 + 4 * 2)]
+    assert bytes(v.ravel()) == data
+    p.write_bytes(data + b"\xff" * 9)
+    assert bytes(load_raw_volume(str(p), size).ravel()) == data
+    p.write_bytes(data[:41])
+    v = load_raw_volume(str(p), size).ravel()
+    assert bytes(v[:41]) == data[:41] and not v[41:].any()
+    with pytest.raises(OSError):
+        load_raw_volume(str(tmp_path / "missing.raw"), size)
+    with pytest.raises(ValueError):
+        load_raw_volume(str(p), (0, 4, 3))

@@ -24,6 +24,18 @@ def test_volume_small_bit_exact(lattice):
     assert n > 0
 
 
+@pytest.mark.parametrize("lattice", [(10, 10, 2), (1500, 11, 7), (11, 900, 5), (12, 12, 700), (300, 300, 3),
+                                     (16384, 10, 2), (10, 16384, 2), (10, 10, 16384)])
+def test_volume_extreme_shapes_bit_exact(lattice):
+    # the smallest lattice taken (9 x 9 x 1 cells), long thin lattices up to
+    # the reference's 16384-sample limit along each axis (init.cpp:82-103),
+    # thousands of slices: tile layouts with one tile row or column, wide and
+    # narrow classification bands, many z segments
+    f = _smooth_noise(lattice, 3)
+    res = _gpu(lattice, 0.5, volume=f)
+    assert _check(res, f, 0.5, lattice) > 100
+
+
 def test_volume_grid_smaller_than_lattice_and_extent():
     lattice, grid, ext = (48, 40, 36), (30, 25, 33), (2.0, 0.5, 1.5)
     f = _smooth_noise(lattice, 11)

@@ -47,6 +47,16 @@ def test_unorm8_equals_fp32_volume(lattice, T):
     assert torch.equal(v8, vf)
 
 
+@pytest.mark.parametrize("lattice", [(16384, 10, 2), (11, 900, 5), (12, 12, 700)])
+def test_unorm8_extreme_shapes(lattice):
+    # byte classification over the longest rows and the most slices
+    b = _bytes(lattice, 15)
+    v8, ce8, c8 = _run(lattice, torch.from_numpy(b).cuda(), 0.5)
+    vf, cef, cf = _run(lattice, torch.from_numpy(_as_float(b)).cuda(), 0.5)
+    assert v8.shape[0] > 100
+    assert torch.equal(c8, cf) and torch.equal(ce8, cef) and torch.equal(v8, vf)
+
+
 def test_unorm8_binary_equals_fp32_volume():
     lattice = (50, 60, 45)
     b = _bytes(lattice, 12)
