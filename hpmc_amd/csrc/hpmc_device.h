@@ -1095,7 +1095,9 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
         *(uint32_t*)&ws.code[4 * lane] = codes4;
         *(uint2*)&ws.cexc[4 * lane] = make_uint2(e0 | (e1 << 16), e2 | (e3 << 16));
         ((uint32_t*)ws.vcell)[lane] = 0u;
-        const uint32_t nv = min(__builtin_amdgcn_readlane(inc, 63), cnt);   // whole-block total
+        // whole-block total (both operands u32: a mixed-sign min would go
+        // through double)
+        const uint32_t nv = min((uint32_t)__builtin_amdgcn_readlane(inc, 63), cnt);
         const uint32_t vhi = min(vlo + (uint32_t)kEmitChunk, nv);
         wave_lds_sync();
         // vertex -> cell for this chunk: each non-empty cell marks its first
