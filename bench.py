@@ -140,11 +140,15 @@ def measure_batched(reps, count=256, iso=1.0):
         b.counts()
 
     def timeit(fn):
+        # each call ends with a host wait, so calls are timed one by one and
+        # the median is reported (a mean of a few calls moves with one stall)
         fn()
-        t0 = time.perf_counter()
+        ts = []
         for _ in range(reps):
+            t0 = time.perf_counter()
             fn()
-        return (time.perf_counter() - t0) * 1e3 / reps
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(sorted(ts)[len(ts) // 2])
 
     res = {
         "workload": "%d metaball FP32 volumes, axes U{16..128}, 4-32 balls, seed 0, iso %g" % (count, iso),
@@ -209,14 +213,15 @@ def copy_calibration(steps):
                       "rw_GBps": 2 * x.numel() * 4 / (ms * 1e-3) / 1e9}))
 
 
-def measure_pipelined(n, steps, warmup, inflight=3, zseg=16):
+def measure_pipelined(n, steps, warmup, inflight=3, zseg=0):
     """Frames in flight: `inflight` HistoPyramids on their own HIP streams,
     each over its own copy of the volume (no cache reuse between frames),
     called round robin -- frame k's extraction overlaps frame k+1's
-    classification.  Every frame is a full build + extract.  Classification
-    in z segments of 16 slices (twice the workgroups of the one-stream
-    default) interleaves better with the other streams' kernels.  Returns the
-    steady-state time per frame (median of 5 runs of `steps` frames)."""
+    classification.  Every frame is a full build + extract.  zseg 0 keeps
+    the one-stream launch shapes (at 512^3: 16-wave classification
+    workgroups over 32-slice segments, which measured 131 us per frame
+    against 134-135 for 16-slice segments).  Returns the steady-state time
+    per frame (median of 5 runs of `steps` frames)."""
     import torch
     import hpmc_amd
     dev = torch.device("cuda", 0)
@@ -323,7 +328,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched/plain comparison")
-    ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--batch-reps", type=int, default=21)
     ap.add_argument("--no-pipeline", action="store_true", help="skip the frames-in-flight measurement")
     ap.add_argument("--no-unorm8", action="store_true", help="skip the 8-bit volume measurement")
     ap.add_argument("--batch-trace", action="store_true",
