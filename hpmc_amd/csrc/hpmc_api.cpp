@@ -595,8 +595,9 @@ bool setup(HPMCHistoPyramid* h)
     }
     spec.field = h->mode == kFieldVolume ? volume_ptr(h) : nullptr;
     std::string err;
-    const int32_t rc = hpmc_host::plan_create(h->plan, {spec}, h->tuning,
-                                              (const void*)hpmc_classify_volume, device_shape(), &err);
+    const bool fp32_volume = h->mode == kFieldVolume && !h->unorm8 && !h->gradient;
+    const int32_t rc = hpmc_host::plan_create(h->plan, {spec}, h->tuning, (const void*)hpmc_classify_volume,
+                                              device_shape(), fp32_volume, &err);
     if (rc != HPMC_OK) {
         fail(h, rc, "%s", err.c_str());
         return false;
@@ -1267,8 +1268,9 @@ int32_t batch_setup(HPMCBatch* b)
     if (!upload_tables(b->constants)) return fail(nullptr, HPMC_ERR_HIP, "no HIP device for the marching-cubes tables");
     batch_drop_graph(b);
     std::string err;
+    // batches hold FP32 volumes
     const int32_t rc = hpmc_host::plan_create(b->plan, b->specs, b->tuning, (const void*)hpmc_classify_volume,
-                                              device_shape(), &err);
+                                              device_shape(), true, &err);
     if (rc != HPMC_OK) return fail(nullptr, rc, "%s", err.c_str());
     b->tainted = false;
     b->built = false;

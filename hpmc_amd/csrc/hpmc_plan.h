@@ -217,10 +217,12 @@ struct DeviceShape {
     } while (0)
 
 // Lays out and allocates a plan for `specs`.  `classify_kernel` is the
-// kernel whose occupancy sizes the automatic zseg.  Returns HPMC_OK or an
-// error code with *err set (the plan is then empty).
+// kernel whose occupancy sizes the automatic zseg; `stack_bands` allows the
+// automatic shape to stack row bands of 4- and 8-wave rows (FP32 volumes).
+// Returns HPMC_OK or an error code with *err set (the plan is then empty).
 inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const Tuning& tune,
-                           const void* classify_kernel, const DeviceShape& dev, std::string* err)
+                           const void* classify_kernel, const DeviceShape& dev, bool stack_bands,
+                           std::string* err)
 {
     plan_free(p);
     if (specs.empty() || specs.size() > 65535) {
@@ -242,13 +244,13 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
         max_row_waves = std::max(max_row_waves, row_waves(specs[i].size[0]));
     }
     // one workgroup shape per launch unless tuned: whole rows of the widest
-    // volume, at least 4 waves (narrower volumes stack row bands); rows of
-    // exactly 4 or 8 waves (X in 193..256 or 449..512) stack bands up to 16
-    // waves, which measured 5-20% faster there (256^3: 40.7 -> 33.4 us,
-    // 512^3: 97.5 -> 92.7 us per classification) and slower for rows of 6
-    // waves (384^3)
+    // volume, at least 4 waves (narrower volumes stack row bands); for FP32
+    // volumes, rows of exactly 4 or 8 waves (X in 193..256 or 449..512)
+    // stack bands up to 16 waves, which measured 5-20% faster there (256^3:
+    // 40.7 -> 33.4 us, 512^3: 97.5 -> 92.7 us per classification), slower
+    // for rows of 6 waves (384^3) and for 8-bit volumes (512^3: 65 -> 70 us)
     int auto_waves = std::max(max_row_waves, 4);
-    if (max_row_waves == 4 || max_row_waves == 8) auto_waves = hpmc::kClassifyMaxWaves;
+    if (stack_bands && (max_row_waves == 4 || max_row_waves == 8)) auto_waves = hpmc::kClassifyMaxWaves;
     p.cls_waves = tune.classify_waves > 0 ? std::min(tune.classify_waves, (int)hpmc::kClassifyMaxWaves)
                                           : auto_waves;
     int per_cu = 0;
