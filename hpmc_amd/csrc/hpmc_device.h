@@ -1126,30 +1126,67 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
         }
         wave_lds_sync();
 
-        // batches of 64 consecutive vertices, one per lane (lanes past the
-        // chunk's end repeat its last vertex and store nothing)
-        for (uint32_t v0 = vlo; v0 < vhi; v0 += 64) {
-            const uint32_t v = min(v0 + (uint32_t)lane, vhi - 1u);
-            const int c = ws.vcell[v - vlo];
-            const uint32_t code = ws.code[c];
-            const uint32_t slot = v - ws.cexc[c];
-            const uint32_t e = (uint32_t)(cases[code] >> (4 + 4 * slot)) & 0xfu;
+        // Output vertex v of the item (vlo <= v < vhi): its cell c (Morton
+        // index in the block) and edge e, the edge's first corner (ax, ay,
+        // az) in brick coordinates and its axis.
+        struct Site {
+            int c, axis, ccx, ccy, ax, ay, az;
+            uint32_t code, slot, e;
+        };
+        auto site_of = [&](uint32_t v) {
+            Site s;
+            s.c = ws.vcell[v - vlo];
+            s.code = ws.code[s.c];
+            s.slot = v - ws.cexc[s.c];
+            s.e = (uint32_t)(cases[s.code] >> (4 + 4 * s.slot)) & 0xfu;
             // edge e = 4 axis + s0 + 2 s1 -> corner offset (dx, dy, dz) of
             // its first end and the axis, bits 0..4 of a 5-bit entry
             constexpr uint64_t kEdgeOffsets =
                 (0x00ull << 0) | (0x02ull << 5) | (0x04ull << 10) | (0x06ull << 15) |     // axis x: (0,s0,s1)
                 (0x08ull << 20) | (0x09ull << 25) | (0x0cull << 30) | (0x0dull << 35) |   // axis y: (s0,0,s1)
                 (0x10ull << 40) | (0x11ull << 45) | (0x12ull << 50) | (0x13ull << 55);    // axis z: (s0,s1,0)
-            const uint32_t eo = (uint32_t)(kEdgeOffsets >> (5 * e)) & 31u;
-            const int axis = (int)(eo >> 3);
-            const int ccx = (int)morton_compact((uint32_t)c), ccy = (int)morton_compact((uint32_t)c >> 1);
-            const int ax = ccx + (int)(eo & 1u), ay = ccy + (int)((eo >> 1) & 1u), az = (int)((eo >> 2) & 1u);
-            const int bx_ = ax + (axis == 0), by_ = ay + (axis == 1), bz_ = az + (axis == 2);
-            const float gx = (float)(cx0 + ax), gy = (float)(cy0 + ay), gz = (float)(cz + az);
+            const uint32_t eo = (uint32_t)(kEdgeOffsets >> (5 * s.e)) & 31u;
+            s.axis = (int)(eo >> 3);
+            s.ccx = (int)morton_compact((uint32_t)s.c);
+            s.ccy = (int)morton_compact((uint32_t)s.c >> 1);
+            s.ax = s.ccx + (int)(eo & 1u);
+            s.ay = s.ccy + (int)((eo >> 1) & 1u);
+            s.az = (int)((eo >> 2) & 1u);
+            return s;
+        };
+        // The staged samples of continuous emission: the edge's ends a and b
+        // and their forward neighbours na = f(a + e_q), nb = f(b + e_q).
+        // Sample (x, y, plane) sits at x + 18 y + 384 slot of the staged
+        // brick; a and b lie in planes 0 or 1, their +z neighbours in planes
+        // 1 or 2 (an edge along z starts in plane 0).
+        struct EdgeSamples {
+            float va, vb, nax, nay, naz, nbx, nby, nbz;
+        };
+        auto samples_of = [&](const Site& s) {
+            const int bx_ = s.ax + (s.axis == 0), by_ = s.ay + (s.axis == 1), bz_ = s.az + (s.axis == 2);
+            const float* Pw = ws.plane[0];
+            const int o0 = s0 * kPlaneStride, o1 = s1 * kPlaneStride, o2 = s2 * kPlaneStride;
+            const int ia = (int)__umul24((uint32_t)s.ay, (uint32_t)kBrick) + s.ax;
+            const int ib = (int)__umul24((uint32_t)by_, (uint32_t)kBrick) + bx_;
+            const int pa = s.az ? o1 : o0, pa1 = s.az ? o2 : o1, pb = bz_ ? o1 : o0, pb1 = bz_ ? o2 : o1;
+            EdgeSamples q;
+            q.va = Pw[ia + pa];
+            q.vb = Pw[ib + pb];
+            q.nax = Pw[ia + pa + 1];
+            q.nay = Pw[ia + pa + kBrick];
+            q.naz = Pw[ia + pa1];
+            q.nbx = Pw[ib + pb + 1];
+            q.nby = Pw[ib + pb + kBrick];
+            q.nbz = Pw[ib + pb1];
+            return q;
+        };
+        // Normal (o[0..2]) and position (o[3..5]) of a vertex, the
+        // reference's emission formulas (shadergen.cpp:505-563).
+        auto eval = [&](const Site& s, float (&o)[6]) {
             float nx, ny, nz, tt;
             if constexpr (MODE == kEmitBinary) {
                 // shadergen.cpp:507-510, 521-522: midpoint, tabulated normal
-                const float4 bn = bnorm[ws.code[c] * 16u + slot];
+                const float4 bn = bnorm[s.code * 16u + s.slot];
                 tt = 0.5f;
                 nx = bn.x;
                 ny = bn.y;
@@ -1157,7 +1194,8 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
             } else if constexpr (MODE == kEmitGradient) {
                 // shadergen.cpp:539-549: values and gradients at both ends
                 const auto field = mk(dc);
-                const float4 fa = field.grad(cx0 + ax, cy0 + ay, cz + az);
+                const int bx_ = s.ax + (s.axis == 0), by_ = s.ay + (s.axis == 1), bz_ = s.az + (s.axis == 2);
+                const float4 fa = field.grad(cx0 + s.ax, cy0 + s.ay, cz + s.az);
                 const float4 fb = field.grad(cx0 + bx_, cy0 + by_, cz + bz_);
                 tt = (fa.w - T) * __builtin_amdgcn_rcpf(fa.w - fb.w);
                 const float u = 1.0f - tt;
@@ -1167,97 +1205,36 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                 ny = T - __builtin_fmaf(fa.y, u, fb.y * tt);
                 nz = T - __builtin_fmaf(fa.z, u, fb.z * tt);
             } else {
-                // sample (x, y, plane) at x + 18 y + 384 slot of the staged
-                // brick; a and b lie in planes 0 or 1, their +z neighbours in
-                // planes 1 or 2 (an edge along z starts in plane 0)
-                const float* Pw = ws.plane[0];
-                const int o0 = s0 * kPlaneStride, o1 = s1 * kPlaneStride, o2 = s2 * kPlaneStride;
-                const int ia = (int)__umul24((uint32_t)ay, (uint32_t)kBrick) + ax;
-                const int ib = (int)__umul24((uint32_t)by_, (uint32_t)kBrick) + bx_;
-                const int pa = az ? o1 : o0, pa1 = az ? o2 : o1, pb = bz_ ? o1 : o0, pb1 = bz_ ? o2 : o1;
-                const float va = Pw[ia + pa], vb = Pw[ib + pb];
-                const float nax = Pw[ia + pa + 1], nay = Pw[ia + pa + kBrick], naz = Pw[ia + pa1];
-                const float nbx_ = Pw[ib + pb + 1], nby_ = Pw[ib + pb + kBrick], nbz_ = Pw[ib + pb1];
+                const EdgeSamples q = samples_of(s);
                 // va and vb lie on opposite sides of T, so va - vb != 0; the
                 // hardware reciprocal (1 ulp) keeps t within a few ulp
-                tt = (va - T) * __builtin_amdgcn_rcpf(va - vb);
+                tt = (q.va - T) * __builtin_amdgcn_rcpf(q.va - q.vb);
                 const float u = 1.0f - tt;
-                if constexpr (MODE == kEmitAdjoint) {
-                    // p = sc (a + t axis), n = sc (T - (na (1-t) + nb t)),
-                    // t = (va - T) / (va - vb): chain rule back to the
-                    // samples va, vb, na_q = f(a + e_q), nb_q = f(b + e_q)
-                    // (na_axis is vb's sample) and to T
-                    const uint32_t o = base + v;
-                    if (v0 + (uint32_t)lane >= vhi || o >= capacity) continue;
-                    const float2* gi = (const float2*)(adj.gout + (size_t)o * 6);
-                    const float2 g01 = gi[0], g23 = gi[1], g45 = gi[2];
-                    const float wx = g01.x * g.scx, wy = g01.y * g.scy, wz = g23.x * g.scz;
-                    float gt = axis == 0 ? g23.y * g.scx : (axis == 1 ? g45.x * g.scy : g45.y * g.scz);
-                    gt += wx * (nax - nbx_) + wy * (nay - nby_) + wz * (naz - nbz_);
-                    const float rD = __builtin_amdgcn_rcpf(va - vb);
-                    const float dva = gt * (T - vb) * rD * rD;
-                    const float dvb = gt * (va - T) * rD * rD;
-                    const float dT = (wx + wy + wz) - gt * rD;
-                    const float dnax = -wx * u, dnay = -wy * u, dnaz = -wz * u;
-                    const auto field = mk(dc);
-                    const int Ax = cx0 + ax, Ay = cy0 + ay, Az = cz + az;
-                    const int Bx = cx0 + bx_, By = cy0 + by_, Bz = cz + bz_;
-                    const uint32_t lb = dc.lattice_base;
-                    uint4 k0, k1;
-                    float4 v0, v1;
-                    k0.x = lb + field.index(Ax, Ay, Az);
-                    v0.x = dva;
-                    k0.y = lb + field.index(Bx, By, Bz);
-                    v0.y = dvb + (axis == 0 ? dnax : (axis == 1 ? dnay : dnaz));
-                    // the two non-axis forward samples of a
-                    const int q0 = axis == 0 ? 1 : 0, q1 = axis == 2 ? 1 : 2;
-                    k0.z = lb + field.index(Ax + (q0 == 0), Ay + (q0 == 1), Az);
-                    v0.z = q0 == 0 ? dnax : dnay;
-                    k0.w = lb + field.index(Ax, Ay + (q1 == 1), Az + (q1 == 2));
-                    v0.w = q1 == 1 ? dnay : dnaz;
-                    k1.x = lb + field.index(Bx + 1, By, Bz);
-                    v1.x = -wx * tt;
-                    k1.y = lb + field.index(Bx, By + 1, Bz);
-                    v1.y = -wy * tt;
-                    k1.z = lb + field.index(Bx, By, Bz + 1);
-                    v1.z = -wz * tt;
-                    k1.w = kAdjointNoKey;
-                    v1.w = 0.0f;
-                    uint4* kd = (uint4*)(adj.keys + (size_t)o * 8);
-                    float4* vd = (float4*)(adj.vals + (size_t)o * 8);
-                    kd[0] = k0;
-                    kd[1] = k1;
-                    vd[0] = v0;
-                    vd[1] = v1;
-                    adj.dT[o] = dT;
-                    continue;
-                }
-                // explicit fma: the same bits however the code is compiled
-                // (ahead of time, or by hipRTC with contraction off)
-                nx = T - __builtin_fmaf(nax, u, nbx_ * tt);
-                ny = T - __builtin_fmaf(nay, u, nby_ * tt);
-                nz = T - __builtin_fmaf(naz, u, nbz_ * tt);
+                nx = T - __builtin_fmaf(q.nax, u, q.nbx * tt);
+                ny = T - __builtin_fmaf(q.nay, u, q.nby * tt);
+                nz = T - __builtin_fmaf(q.naz, u, q.nbz * tt);
             }
-            if constexpr (CB::kActive) {
-                // a user-fused traversal: hand the vertex over, write nothing
-                if (v0 + (uint32_t)lane < vhi)
-                    cb(base + v, make_float3(nx * g.scx, ny * g.scy, nz * g.scz),
-                       make_float3((gx + (axis == 0 ? tt : 0.0f)) * g.scx, (gy + (axis == 1 ? tt : 0.0f)) * g.scy,
-                                   (gz + (axis == 2 ? tt : 0.0f)) * g.scz));
-                continue;
-            }
-            // The batch's 64 x 24 bytes go out as three contiguous 512-byte
-            // stores (lane L writes floats 2(64k + L), +1) after a transpose
-            // through LDS, non-temporal: the output streams to memory instead
-            // of staying dirty in L2, where its write-back would compete with
-            // the next build's classification.
-            typedef float v2f __attribute__((ext_vector_type(2)));
-            typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+            const float gx = (float)(cx0 + s.ax), gy = (float)(cy0 + s.ay), gz = (float)(cz + s.az);
+            o[0] = nx * g.scx;
+            o[1] = ny * g.scy;
+            o[2] = nz * g.scz;
+            o[3] = (gx + (s.axis == 0 ? tt : 0.0f)) * g.scx;
+            o[4] = (gy + (s.axis == 1 ? tt : 0.0f)) * g.scy;
+            o[5] = (gz + (s.axis == 2 ? tt : 0.0f)) * g.scz;
+        };
+        // The batch of 64 vertices v0 .. v0 + 63 (clipped at vhi) goes out
+        // as three contiguous 512-byte stores (lane L writes floats
+        // 2(64k + L), +1) after a transpose through LDS, non-temporal: the
+        // output streams to memory instead of staying dirty in L2, where its
+        // write-back would compete with the next build's classification.
+        typedef float v2f __attribute__((ext_vector_type(2)));
+        typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+        auto store_batch = [&](uint32_t v0, const Site& s, const float (&o)[6]) {
             {
                 v2f* st = (v2f*)(ws.ostage + 6 * lane);
-                st[0] = (v2f){nx * g.scx, ny * g.scy};
-                st[1] = (v2f){nz * g.scz, (gx + (axis == 0 ? tt : 0.0f)) * g.scx};
-                st[2] = (v2f){(gy + (axis == 1 ? tt : 0.0f)) * g.scy, (gz + (axis == 2 ? tt : 0.0f)) * g.scz};
+                st[0] = (v2f){o[0], o[1]};
+                st[1] = (v2f){o[2], o[3]};
+                st[2] = (v2f){o[4], o[5]};
             }
             const uint32_t start = base + v0;   // first output vertex of the batch
             const uint32_t nval = min(vhi - v0, 64u);
@@ -1271,9 +1248,85 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
             }
             wave_lds_sync();
             if (out_ce && v0 + (uint32_t)lane < vhi && start + lane < capacity) {
-                const uint32_t cell = (uint32_t)(cx0 + ccx) +
-                                      (uint32_t)g.cx * ((uint32_t)(cy0 + ccy) + (uint32_t)g.cy * (uint32_t)cz);
-                __builtin_nontemporal_store((v2u){cell, e}, (v2u*)(out_ce + start + lane));
+                const uint32_t cell = (uint32_t)(cx0 + s.ccx) +
+                                      (uint32_t)g.cx * ((uint32_t)(cy0 + s.ccy) + (uint32_t)g.cy * (uint32_t)cz);
+                __builtin_nontemporal_store((v2u){cell, s.e}, (v2u*)(out_ce + start + lane));
+            }
+        };
+
+        // batches of 64 consecutive vertices, one per lane (lanes past the
+        // chunk's end repeat its last vertex and store nothing)
+        if constexpr (MODE == kEmitAdjoint) {
+            for (uint32_t v0 = vlo; v0 < vhi; v0 += 64) {
+                const uint32_t v = min(v0 + (uint32_t)lane, vhi - 1u);
+                const uint32_t o = base + v;
+                if (v0 + (uint32_t)lane >= vhi || o >= capacity) continue;
+                const Site s = site_of(v);
+                const EdgeSamples q = samples_of(s);
+                const int axis = s.axis;
+                // p = sc (a + t axis), n = sc (T - (na (1-t) + nb t)),
+                // t = (va - T) / (va - vb): chain rule back to the samples
+                // va, vb, na_q = f(a + e_q), nb_q = f(b + e_q) (na_axis is
+                // vb's sample) and to T
+                const float tt = (q.va - T) * __builtin_amdgcn_rcpf(q.va - q.vb);
+                const float u = 1.0f - tt;
+                const float2* gi = (const float2*)(adj.gout + (size_t)o * 6);
+                const float2 g01 = gi[0], g23 = gi[1], g45 = gi[2];
+                const float wx = g01.x * g.scx, wy = g01.y * g.scy, wz = g23.x * g.scz;
+                float gt = axis == 0 ? g23.y * g.scx : (axis == 1 ? g45.x * g.scy : g45.y * g.scz);
+                gt += wx * (q.nax - q.nbx) + wy * (q.nay - q.nby) + wz * (q.naz - q.nbz);
+                const float rD = __builtin_amdgcn_rcpf(q.va - q.vb);
+                const float dva = gt * (T - q.vb) * rD * rD;
+                const float dvb = gt * (q.va - T) * rD * rD;
+                const float dT = (wx + wy + wz) - gt * rD;
+                const float dnax = -wx * u, dnay = -wy * u, dnaz = -wz * u;
+                const auto field = mk(dc);
+                const int Ax = cx0 + s.ax, Ay = cy0 + s.ay, Az = cz + s.az;
+                const int Bx = Ax + (axis == 0), By = Ay + (axis == 1), Bz = Az + (axis == 2);
+                const uint32_t lb = dc.lattice_base;
+                uint4 k0, k1;
+                float4 w0, w1;
+                k0.x = lb + field.index(Ax, Ay, Az);
+                w0.x = dva;
+                k0.y = lb + field.index(Bx, By, Bz);
+                w0.y = dvb + (axis == 0 ? dnax : (axis == 1 ? dnay : dnaz));
+                // the two non-axis forward samples of a
+                const int q0 = axis == 0 ? 1 : 0, q1 = axis == 2 ? 1 : 2;
+                k0.z = lb + field.index(Ax + (q0 == 0), Ay + (q0 == 1), Az);
+                w0.z = q0 == 0 ? dnax : dnay;
+                k0.w = lb + field.index(Ax, Ay + (q1 == 1), Az + (q1 == 2));
+                w0.w = q1 == 1 ? dnay : dnaz;
+                k1.x = lb + field.index(Bx + 1, By, Bz);
+                w1.x = -wx * tt;
+                k1.y = lb + field.index(Bx, By + 1, Bz);
+                w1.y = -wy * tt;
+                k1.z = lb + field.index(Bx, By, Bz + 1);
+                w1.z = -wz * tt;
+                k1.w = kAdjointNoKey;
+                w1.w = 0.0f;
+                uint4* kd = (uint4*)(adj.keys + (size_t)o * 8);
+                float4* vd = (float4*)(adj.vals + (size_t)o * 8);
+                kd[0] = k0;
+                kd[1] = k1;
+                vd[0] = w0;
+                vd[1] = w1;
+                adj.dT[o] = dT;
+            }
+        } else if constexpr (CB::kActive) {
+            // a user-fused traversal: hand each vertex over, write nothing
+            for (uint32_t v0 = vlo; v0 < vhi; v0 += 64) {
+                const uint32_t v = min(v0 + (uint32_t)lane, vhi - 1u);
+                float o[6];
+                eval(site_of(v), o);
+                if (v0 + (uint32_t)lane < vhi)
+                    cb(base + v, make_float3(o[0], o[1], o[2]), make_float3(o[3], o[4], o[5]));
+            }
+        } else {
+            for (uint32_t v0 = vlo; v0 < vhi; v0 += 64) {
+                const Site s = site_of(min(v0 + (uint32_t)lane, vhi - 1u));
+                float o[6];
+                eval(s, o);
+                store_batch(v0, s, o);
             }
         }
         if (!more) break;
