@@ -266,22 +266,14 @@ def measure_pipelined(n, steps, warmup, inflight=3, zseg=0):
                     "steady-state interval, not the latency of one frame"}
 
 
-def measure_unorm8(n, steps, warmup):
-    """The Cayley field quantised to an 8-bit normalised volume (byte
-    clamp(round(128 + 64 f)), iso 128/255: the same surface to within the
-    quantisation), build + extract on one stream; 1 byte per lattice point."""
+def time_build_extract(iso, T, steps, warmup):
+    """Vertex count, wall ms per build + extract and per-stage ms (HIP events
+    on the library's stream, a separate pass) of a configured IsoSurface,
+    output bound to a buffer of the vertex count."""
     import torch
-    import hpmc_amd
-    dev = torch.device("cuda", 0)
-    f = cayley_volume_torch(n, dev)
-    vol = torch.clamp(torch.round(128.0 + 64.0 * f), 0, 255).to(torch.uint8).contiguous()
-    del f
-    T = 128.0 / 255.0
-    iso = hpmc_amd.IsoSurface((n, n, n))
-    iso.set_volume(vol)
     iso.build(T)
     nv = iso.count()
-    out = torch.empty((max(nv, 1), 6), dtype=torch.float32, device=dev)
+    out = torch.empty((max(nv, 1), 6), dtype=torch.float32, device=torch.device("cuda", 0))
     iso.bind_output(out)
     for _ in range(warmup):
         iso.build(T)
@@ -301,11 +293,63 @@ def measure_unorm8(n, steps, warmup):
         iso.extract()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
+    return nv, ms, {k: v[0] / max(v[1], 1) for k, v in st.items()}
+
+
+def measure_unorm8(n, steps, warmup):
+    """The Cayley field quantised to an 8-bit normalised volume (byte
+    clamp(round(128 + 64 f)), iso 128/255: the same surface to within the
+    quantisation), build + extract on one stream; 1 byte per lattice point."""
+    import torch
+    import hpmc_amd
+    dev = torch.device("cuda", 0)
+    f = cayley_volume_torch(n, dev)
+    vol = torch.clamp(torch.round(128.0 + 64.0 * f), 0, 255).to(torch.uint8).contiguous()
+    del f
+    iso = hpmc_amd.IsoSurface((n, n, n))
+    iso.set_volume(vol)
+    nv, ms, stage_ms = time_build_extract(iso, 128.0 / 255.0, steps, warmup)
     iso.close()
-    stage_ms = {k: v[0] / max(v[1], 1) for k, v in st.items()}
     return {"workload": "cayley-%d^3 quantised to uint8 (128 + 64 f), iso 128/255" % n, "vertices": nv,
             "ms_per_step": ms, "cells_per_s": (n - 1) ** 3 / (ms * 1e-3), "stage_ms": stage_ms,
             "classify_GBps": n ** 3 / (stage_ms["classify"] * 1e-3) / 1e9}
+
+
+def measure_snippet(n, steps, warmup):
+    """The Cayley cubic as a hipRTC field snippet (no volume in memory),
+    iso 0: classification evaluates it at every lattice point and emission at
+    the samples of the bricks it extracts, so the bound is the VALU, not
+    HBM; reported as cells/s and verts/s."""
+    import hpmc_amd
+    from hpmc_amd import workloads
+    iso = hpmc_amd.IsoSurface((n, n, n))
+    iso.set_snippet(workloads.CAYLEY_SNIPPET)
+    nv, ms, stage_ms = time_build_extract(iso, 0.0, steps, warmup)
+    iso.close()
+    return {"workload": "cayley-%d^3 hipRTC snippet, iso 0, positions+normals" % n, "vertices": nv,
+            "ms_per_step": ms, "cells_per_s": (n - 1) ** 3 / (ms * 1e-3), "verts_per_s": nv / (ms * 1e-3),
+            "stage_ms": stage_ms}
+
+
+def measure_texture256(steps, warmup):
+    """256^3 FP32 volume: a seeded uniform [0,1] 32^3 grid upsampled
+    trilinearly (corner-aligned), iso 0.5, build + extract on one stream."""
+    import torch
+    import hpmc_amd
+    from hpmc_amd import workloads
+    n = 256
+    vol = torch.from_numpy(workloads.upsampled_noise(n)).cuda()
+    iso = hpmc_amd.IsoSurface((n, n, n))
+    iso.set_volume(vol)
+    nv, ms, stage_ms = time_build_extract(iso, 0.5, steps, warmup)
+    tuning = iso.tuning()
+    iso.close()
+    algo = 4 * n ** 3 + 24 * nv
+    return {"workload": "256^3 fp32 volume, seeded 32^3 uniform noise upsampled trilinearly, iso 0.5",
+            "vertices": nv, "ms_per_step": ms, "cells_per_s": (n - 1) ** 3 / (ms * 1e-3),
+            "verts_per_s": nv / (ms * 1e-3), "hbm_frac_step": algo / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+            "stage_ms": stage_ms,
+            "tuning": {"classify_waves": tuning[0], "zseg": tuning[1], "emit_workgroups": tuning[2]}}
 
 
 def traffic():
@@ -331,6 +375,8 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=21)
     ap.add_argument("--no-pipeline", action="store_true", help="skip the frames-in-flight measurement")
     ap.add_argument("--no-unorm8", action="store_true", help="skip the 8-bit volume measurement")
+    ap.add_argument("--no-snippet", action="store_true", help="skip the hipRTC Cayley snippet measurement")
+    ap.add_argument("--no-texture256", action="store_true", help="skip the 256^3 noise volume measurement")
     ap.add_argument("--batch-trace", action="store_true",
                     help="only the batched build + extract at a fixed launch shape (for kernel traces)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -428,6 +474,10 @@ def main():
         line["pipelined"] = pl
     if not args.no_unorm8:
         line["unorm8"] = measure_unorm8(n, args.steps, args.warmup)
+    if not args.no_snippet:
+        line["snippet"] = measure_snippet(n, args.steps, args.warmup)
+    if not args.no_texture256:
+        line["texture256"] = measure_texture256(args.steps, args.warmup)
     if not args.no_batch:
         line["batched"] = measure_batched(args.batch_reps)
     if not args.no_cpu_baseline:
