@@ -1033,6 +1033,13 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     __syncthreads();
     if (!has_items) return;
 
+    // Per lane, the vertex counts and first vertices of its four cells and
+    // the block's vertex total, kept from item to item: a further chunk of
+    // the same block slice (the next item, as the scan lays them out) finds
+    // its cells' codes and first vertices in the wave's LDS already and
+    // skips classification.
+    uint32_t cc[4] = {0u, 0u, 0u, 0u}, ex[4] = {0u, 0u, 0u, 0u}, blk_total = 0u;
+    bool reuse = false;
     for (;;) {
         const uint32_t vlo = item_chunk_first(cur.y), cnt = item_block_count(cur.y);
         const uint32_t base = obase;
@@ -1047,7 +1054,9 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
         // next item: entry, output offset and missing planes in flight meanwhile
         const bool more = it + 1 < iend;
         uint2 after = nxt;
+        bool reuse_next = false;
         if (more) {
+            reuse_next = nxt.x == cur.x && (!BATCH || item_volume(nxt.y) == cvol);
             if constexpr (BATCH) dn = load_uniform(vols + item_volume(nxt.y));
             obase = load_uniform(l4_off + dn.l4_base + nxt.x);
             block_origin(dn.g, nxt.x, x0, y0, z);
@@ -1061,43 +1070,47 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
         const int s0 = cz % 3, s1 = (cz + 1) % 3, s2 = (cz + 2) % 3;
         const float* P0 = ws.plane[s0];
         const float* P1 = ws.plane[s1];
-        const bool qvx0 = cx0 + qx < g.cx, qvx1 = cx0 + qx + 1 < g.cx;
-        const bool qvy0 = cy0 + qy < g.cy, qvy1 = cy0 + qy + 1 < g.cy;
+        if (!reuse) {
+            const bool qvx0 = cx0 + qx < g.cx, qvx1 = cx0 + qx + 1 < g.cx;
+            const bool qvy0 = cy0 + qy < g.cy, qvy1 = cy0 + qy + 1 < g.cy;
 
-        // classify the quad: corners (qx..qx+2, qy..qy+2) of planes z, z+1
-        uint32_t bits0 = 0, bits1 = 0;   // 3x3 inside bits per plane, bit = dx + 3*dy
+            // classify the quad: corners (qx..qx+2, qy..qy+2) of planes z, z+1
+            uint32_t bits0 = 0, bits1 = 0;   // 3x3 inside bits per plane, bit = dx + 3*dy
 #pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
+            for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                const int i = (qy + dy) * kBrick + qx + dx;
-                bits0 |= (P0[i] < T ? 1u : 0u) << (dx + 3 * dy);
-                bits1 |= (P1[i] < T ? 1u : 0u) << (dx + 3 * dy);
+                for (int dx = 0; dx < 3; ++dx) {
+                    const int i = (qy + dy) * kBrick + qx + dx;
+                    bits0 |= (P0[i] < T ? 1u : 0u) << (dx + 3 * dy);
+                    bits1 |= (P1[i] < T ? 1u : 0u) << (dx + 3 * dy);
+                }
+            uint32_t codes4 = 0, cnt4 = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int cxq = q & 1, cyq = q >> 1;
+                const int sh0 = cxq + 3 * cyq;
+                const uint32_t lo2 = ((bits0 >> sh0) & 3u) | (((bits0 >> (sh0 + 3)) & 3u) << 2);
+                const uint32_t hi2 = ((bits1 >> sh0) & 3u) | (((bits1 >> (sh0 + 3)) & 3u) << 2);
+                const uint32_t code = lo2 | (hi2 << 4);
+                const bool valid = (cxq ? qvx1 : qvx0) && (cyq ? qvy1 : qvy0);
+                cc[q] = (uint32_t)lut[code] & (valid ? 0xffu : 0u);
+                codes4 |= code << (8 * q);
+                cnt4 += cc[q];
             }
-        uint32_t codes4 = 0, cnt4 = 0, cc[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int cxq = q & 1, cyq = q >> 1;
-            const int sh0 = cxq + 3 * cyq;
-            const uint32_t lo2 = ((bits0 >> sh0) & 3u) | (((bits0 >> (sh0 + 3)) & 3u) << 2);
-            const uint32_t hi2 = ((bits1 >> sh0) & 3u) | (((bits1 >> (sh0 + 3)) & 3u) << 2);
-            const uint32_t code = lo2 | (hi2 << 4);
-            const bool valid = (cxq ? qvx1 : qvx0) && (cyq ? qvy1 : qvy0);
-            cc[q] = (uint32_t)lut[code] & (valid ? 0xffu : 0u);
-            codes4 |= code << (8 * q);
-            cnt4 += cc[q];
+            // first vertex of every cell: an inclusive scan of the quad counts
+            // (cells in Morton order = the reference's child order,
+            // shadergen.cpp:448-495, so Morton order is its vertex order)
+            const uint32_t inc = wave_scan_u32<false>(cnt4);
+            ex[0] = inc - cnt4;
+            ex[1] = ex[0] + cc[0];
+            ex[2] = ex[1] + cc[1];
+            ex[3] = ex[2] + cc[2];
+            *(uint32_t*)&ws.code[4 * lane] = codes4;
+            *(uint2*)&ws.cexc[4 * lane] = make_uint2(ex[0] | (ex[1] << 16), ex[2] | (ex[3] << 16));
+            blk_total = (uint32_t)__builtin_amdgcn_readlane(inc, 63);
         }
-        // first vertex of every cell: an inclusive scan of the quad counts
-        // (cells in Morton order = the reference's child order,
-        // shadergen.cpp:448-495, so Morton order is its vertex order)
-        const uint32_t inc = wave_scan_u32<false>(cnt4);
-        const uint32_t e0 = inc - cnt4, e1 = e0 + cc[0], e2 = e1 + cc[1], e3 = e2 + cc[2];
-        *(uint32_t*)&ws.code[4 * lane] = codes4;
-        *(uint2*)&ws.cexc[4 * lane] = make_uint2(e0 | (e1 << 16), e2 | (e3 << 16));
         ((uint32_t*)ws.vcell)[lane] = 0u;
-        // whole-block total (both operands u32: a mixed-sign min would go
-        // through double)
-        const uint32_t nv = min((uint32_t)__builtin_amdgcn_readlane(inc, 63), cnt);
+        const uint32_t nv = min(blk_total, cnt);   // whole-block total
         const uint32_t vhi = min(vlo + (uint32_t)kEmitChunk, nv);
         wave_lds_sync();
         // vertex -> cell for this chunk: each non-empty cell marks its first
@@ -1105,7 +1118,6 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
         // scan over the chunk carries every mark forward
         {
             // marks that do not apply go to this lane's spare byte
-            const uint32_t ex[4] = {e0, e1, e2, e3};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int pos = (int)ex[q] - (int)vlo;
@@ -1331,6 +1343,7 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
         }
         if (!more) break;
         ++it;
+        reuse = reuse_next;
         cur = nxt;
         nxt = after;
     }
