@@ -246,11 +246,13 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
     // one workgroup shape per launch unless tuned: whole rows of the widest
     // volume, at least 4 waves (narrower volumes stack row bands); for FP32
     // volumes, rows of exactly 4 or 8 waves (X in 193..256 or 449..512)
-    // stack bands up to 16 waves, which measured 5-20% faster there (256^3:
-    // 40.7 -> 33.4 us, 512^3: 97.5 -> 92.7 us per classification), slower
-    // for rows of 6 waves (384^3) and for 8-bit volumes (512^3: 65 -> 70 us)
+    // stack two bands, which measured faster there for sparse and dense
+    // surfaces alike (classification of the 512^3 Cayley volume 97.5 ->
+    // 92.7 us, of a 256^3 one 40.7 -> 32.7 us, of a 256^3 volume with a
+    // vertex per cell 82 -> 67 us), and slower for rows of 6 waves (384^3)
+    // and for 8-bit volumes (512^3: 65 -> 70 us)
     int auto_waves = std::max(max_row_waves, 4);
-    if (stack_bands && (max_row_waves == 4 || max_row_waves == 8)) auto_waves = hpmc::kClassifyMaxWaves;
+    if (stack_bands && (max_row_waves == 4 || max_row_waves == 8)) auto_waves = 2 * max_row_waves;
     p.cls_waves = tune.classify_waves > 0 ? std::min(tune.classify_waves, (int)hpmc::kClassifyMaxWaves)
                                           : auto_waves;
     int per_cu = 0;
