@@ -720,13 +720,13 @@ __device__ __forceinline__ void scan_load_tile(const VolDesc& d, const uint16_t*
 // sequence (a few hundred), scans its own 4096 entries and scatters.
 // Level-4 tiles write, for each non-empty block, the global offset of its
 // first vertex: the concatenation of the volumes' vertex lists, each in
-// Morton key order (the reference's key order); a volume's first level-4
-// tile writes its vertex base and end.  Column tiles write the emit work
-// items -- one per kEmitChunk vertices of a block -- in (volume, z segment,
-// y, x, z) order, so that consecutive items are consecutive slices of one
-// block column.  Tile 0 writes the totals, to device memory and to the
-// host-visible copy.  The last workgroup to have read the tile sums clears
-// them for the next build.
+// Morton key order (the reference's key order); a volume's first and last
+// level-4 tiles write its vertex base and end.  Column tiles write the emit
+// work items -- one per kEmitChunk vertices of a block -- in (volume, z
+// segment, y, x, z) order, so that consecutive items are consecutive slices
+// of one block column.  The last level-4 tile and the last column tile write
+// the totals, to device memory and to the host-visible copy.  The last
+// workgroup to have read the tile sums clears them for the next build.
 struct ScanState {
     uint32_t done;            // workgroups done with the tile sums
     uint32_t pad[3];
@@ -766,38 +766,6 @@ __device__ __forceinline__ void scan_body(
         else mine.y += chunks_of(c[k]);
     }
     pre = block_reduce_u2(pre, sh);
-    if (!is_col && local == 0) {
-        // this volume's vertex base and end
-        uint32_t cnt = 0;
-        for (uint32_t i = threadIdx.x; i < d.n_l4tiles; i += blockDim.x) cnt += tile_sums[d.l4_tile0 + i].x;
-        cnt = block_reduce_u2(make_uint2(cnt, 0), sh).x;
-        if (threadIdx.x == 0) {
-            VolCount vc;
-            vc.base = pre.x;
-            vc.end = pre.x + cnt;
-            vol_counts[vol] = vc;
-            if (host_vol_counts) host_vol_counts[vol] = vc;
-        }
-    }
-    if (tile == 0) {
-        uint2 tot = make_uint2(0, 0), items_tot = make_uint2(0, 0);
-        for (uint32_t i = threadIdx.x; i < n_l4t; i += blockDim.x) {
-            tot.x += tile_sums[i].x;
-            tot.y += tile_sums[i].y;
-        }
-        for (uint32_t i = threadIdx.x; i < n_colt; i += blockDim.x) items_tot.y += tile_sums[n_l4t + i].y;
-        tot = block_reduce_u2(tot, sh);
-        items_tot = block_reduce_u2(items_tot, sh);
-        if (threadIdx.x == 0) {
-            Totals t;
-            t.vertices = tot.x;
-            t.blocks = tot.y;
-            t.items = items_tot.y;
-            t.pad = 0;
-            *totals = t;
-            if (host_totals) *host_totals = t;
-        }
-    }
     // This workgroup has read every tile sum it needs (the reductions above
     // consumed them): count it out now, so that the round trip overlaps the
     // scan.  The last workgroup to do so clears the sums for the next build.
@@ -814,6 +782,37 @@ __device__ __forceinline__ void scan_body(
     }
     if (lane == 63) wsum[w] = inc;
     __syncthreads();
+    // A volume's first level-4 tile knows its vertex base, its last one the
+    // end; the last tile of each sequence knows the grand totals (the sums
+    // of the tiles before it plus its own).
+    if (threadIdx.x == 0) {
+        uint2 all = pre;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+            all.x += wsum[i].x;
+            all.y += wsum[i].y;
+        }
+        if (!is_col) {
+            if (local == 0) {
+                vol_counts[vol].base = pre.x;
+                if (host_vol_counts) host_vol_counts[vol].base = pre.x;
+            }
+            if (local == d.n_l4tiles - 1) {
+                vol_counts[vol].end = all.x;
+                if (host_vol_counts) host_vol_counts[vol].end = all.x;
+            }
+            if (tile == n_l4t - 1) {
+                totals->vertices = all.x;
+                totals->blocks = all.y;
+                if (host_totals) {
+                    host_totals->vertices = all.x;
+                    host_totals->blocks = all.y;
+                }
+            }
+        } else if (tile == ntiles - 1) {
+            totals->items = all.y;
+            if (host_totals) host_totals->items = all.y;
+        }
+    }
     uint2 off = make_uint2(pre.x + inc.x - mine.x, pre.y + inc.y - mine.y);
     for (int i = 0; i < w; ++i) {
         off.x += wsum[i].x;
