@@ -352,6 +352,63 @@ def measure_texture256(steps, warmup):
             "tuning": {"classify_waves": tuning[0], "zseg": tuning[1], "emit_workgroups": tuning[2]}}
 
 
+def measure_adjoint(reps, count=64, seed=0):
+    """Differentiation: `count` cubic FP32 volumes, side ~U{32..128}, f = |p -
+    1/2| - 0.3 + N(0, 0.01^2) per lattice point, iso 0; loss = mean over all
+    vertices of (|v - centre| - 0.35)^2, the sphere centre being the position
+    (0.5, 0.5, 0.5) in HPMC's [0, extent] convention.  One persistent batch:
+    forward = build + counts + extract, then dL/dvertices by torch, then the
+    adjoint back to every field and to the iso value.  Medians of `reps`
+    single steps (each ends with a host wait)."""
+    import torch
+    import hpmc_amd
+    from hpmc_amd import workloads
+    dev = torch.device("cuda", 0)
+    rng = workloads.SplitMix64(seed)
+    sides = [rng.randint(32, 128) for _ in range(count)]
+    fields = [torch.from_numpy(workloads.noisy_sphere((n, n, n), seed + 1 + i)).to(dev) for i, n in enumerate(sides)]
+    b = hpmc_amd.Batch()
+    for f in fields:
+        b.add_volume(f)
+    b.build(0.0)
+    _, _, total = b.counts()
+    out = torch.empty((total, 6), dtype=torch.float32, device=dev)
+    centre = torch.full((3,), 0.5, dtype=torch.float32, device=dev)
+
+    def forward():
+        b.build(0.0)
+        b.counts()
+        b.extract(out)
+
+    def loss_grad():
+        v = out.detach().requires_grad_(True)
+        loss = ((v[:, 3:] - centre).norm(dim=1) - 0.35).pow(2).mean()
+        (gv,) = torch.autograd.grad(loss, v)
+        return gv
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(sorted(ts)[len(ts) // 2]), r
+
+    fwd_ms, _ = timed(forward)
+    gv = loss_grad()
+    adj_ms, (grads, gt) = timed(lambda: b.adjoint(gv))
+    step_ms, _ = timed(lambda: (forward(), b.adjoint(loss_grad())))
+    assert torch.isfinite(gt) and all(torch.isfinite(g).all() for g in grads)
+    lattice = sum(n ** 3 for n in sides)
+    b.close()
+    return {"workload": "%d noisy-sphere FP32 volumes, sides U{32..128}, iso 0, loss mean (|v - c| - 0.35)^2" % count,
+            "volumes": count, "lattice_points": lattice, "vertices": int(total),
+            "forward_ms": fwd_ms, "adjoint_ms": adj_ms, "step_ms": step_ms}
+
+
 def traffic():
     """HBM bytes per classify launch from the committed counter profile of this
     workload, or None when the profile is absent."""
@@ -377,6 +434,9 @@ def main():
     ap.add_argument("--no-unorm8", action="store_true", help="skip the 8-bit volume measurement")
     ap.add_argument("--no-snippet", action="store_true", help="skip the hipRTC Cayley snippet measurement")
     ap.add_argument("--no-texture256", action="store_true", help="skip the 256^3 noise volume measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation measurement")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the differentiation measurement (for kernel traces of it)")
     ap.add_argument("--batch-trace", action="store_true",
                     help="only the batched build + extract at a fixed launch shape (for kernel traces)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -389,6 +449,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.steps)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": measure_adjoint(args.batch_reps)}))
         return
     if args.batch_trace:
         batch_steps(args.steps, args.warmup)
@@ -480,6 +543,8 @@ def main():
         line["texture256"] = measure_texture256(args.steps, args.warmup)
     if not args.no_batch:
         line["batched"] = measure_batched(args.batch_reps)
+    if not args.no_adjoint:
+        line["adjoint"] = measure_adjoint(args.batch_reps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_reps)
     print(json.dumps(line))

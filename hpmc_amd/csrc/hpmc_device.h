@@ -1350,18 +1350,46 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
 
 // Adjoint reduction, after the (key, value) pairs are sorted by key with
 // ties in vertex order: each run of equal keys is summed in that order by
-// the thread at its first element (runs are short: a sample feeds at most a
-// few dozen vertices), so the result does not depend on the launch.
+// the lane holding its first element, so the result does not depend on the
+// launch.  Each wave stages a tile of kSegTile pairs in LDS with coalesced
+// loads; lane L takes the runs that start in tile elements [16 L, 16 L + 16)
+// and reads on in global memory when a run crosses the tile's end (runs are
+// short: a sample feeds at most a few dozen vertices).  Launched with 256
+// threads per workgroup.
+constexpr int kSegPer = 16;
+constexpr int kSegTile = 64 * kSegPer;
 __device__ __forceinline__ void adjoint_segments_body(const uint32_t* __restrict__ keys,
                                                       const float* __restrict__ vals, uint32_t n,
                                                       float* __restrict__ grad)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t k = keys[i];
-        if (k == kAdjointNoKey || (i > 0 && keys[i - 1] == k)) continue;
-        float s = 0.0f;
-        for (uint32_t j = i; j < n && keys[j] == k; ++j) s += vals[j];
-        grad[k] = s;
+    __shared__ uint32_t sk[4][kSegTile];
+    __shared__ float sv[4][kSegTile];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    const uint64_t ntiles = ((uint64_t)n + kSegTile - 1) / kSegTile;
+    for (uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + w; t < ntiles; t += nwaves) {
+        const uint64_t base = t * kSegTile;
+#pragma unroll
+        for (int k = 0; k < kSegPer; ++k) {
+            const uint64_t i = base + 64 * k + lane;
+            sk[w][64 * k + lane] = i < n ? keys[i] : kAdjointNoKey;
+            sv[w][64 * k + lane] = i < n ? vals[i] : 0.0f;
+        }
+        // the key before the tile: a run that starts there is summed by the
+        // lane of the tile it starts in
+        const uint32_t kprev = base > 0 ? keys[base - 1] : kAdjointNoKey;
+        wave_lds_sync();
+        for (int e = kSegPer * lane; e < kSegPer * (lane + 1); ++e) {
+            const uint32_t k = sk[w][e];
+            if (k == kAdjointNoKey || k == (e > 0 ? sk[w][e - 1] : kprev)) continue;
+            float sum = 0.0f;
+            int j = e;
+            for (; j < kSegTile && sk[w][j] == k; ++j) sum += sv[w][j];
+            if (j == kSegTile)
+                for (uint64_t g = base + kSegTile; g < n && keys[g] == k; ++g) sum += vals[g];
+            grad[k] = sum;
+        }
+        wave_lds_sync();
     }
 }
 
