@@ -1349,45 +1349,125 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
 }
 
 // Adjoint reduction, after the (key, value) pairs are sorted by key with
-// ties in vertex order: each run of equal keys is summed in that order by
-// the lane holding its first element, so the result does not depend on the
-// launch.  Each wave stages a tile of kSegTile pairs in LDS with coalesced
-// loads; lane L takes the runs that start in tile elements [16 L, 16 L + 16)
-// and reads on in global memory when a run crosses the tile's end (runs are
-// short: a sample feeds at most a few dozen vertices).  Launched with 256
-// threads per workgroup.
+// ties in vertex order: every run of equal keys is summed from 0 in array
+// order, one addition after the other, so the result does not depend on the
+// launch.  A wave takes a tile of kSegTile pairs, 16 consecutive ones per
+// lane in registers.  Each lane walks its 16 pairs once: runs that start
+// and end in them are written at once; the run its pairs end with, if it
+// goes on into the next lane, leaves its running sum there as a carry.  A
+// lane whose 16 pairs all continue the run of the lane before passes the
+// carry on after adding its own (a few rounds at most: a sample feeds at
+// most a few dozen vertices), and the lane where the run ends adds its head
+// pairs to the carry and writes the sum.  Runs arriving from the previous
+// tile belong to it; a run going on past the tile is finished from the
+// next 64 pairs, staged in LDS (and from global memory in the unlikely case
+// that it is longer still).  Launched with 256 threads per workgroup.
 constexpr int kSegPer = 16;
 constexpr int kSegTile = 64 * kSegPer;
+__device__ __forceinline__ uint32_t wave_shr1_or(uint32_t x, uint32_t lane0)
+{
+    const uint32_t y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x138, 0xf, 0xf, false);   // wave_shr:1
+    return (threadIdx.x & 63) == 0 ? lane0 : y;
+}
+__device__ __forceinline__ uint32_t wave_shl1_or(uint32_t x, uint32_t lane63)
+{
+    const uint32_t y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x130, 0xf, 0xf, false);   // wave_shl:1
+    return (threadIdx.x & 63) == 63 ? lane63 : y;
+}
 __device__ __forceinline__ void adjoint_segments_body(const uint32_t* __restrict__ keys,
                                                       const float* __restrict__ vals, uint32_t n,
                                                       float* __restrict__ grad)
 {
-    __shared__ uint32_t sk[4][kSegTile];
-    __shared__ float sv[4][kSegTile];
+    __shared__ uint32_t nk[4][64];   // per wave: the 64 pairs after its tile
+    __shared__ float nv[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
     const uint64_t ntiles = ((uint64_t)n + kSegTile - 1) / kSegTile;
     for (uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + w; t < ntiles; t += nwaves) {
         const uint64_t base = t * kSegTile;
+        const uint64_t mine = base + (uint64_t)kSegPer * lane;
+        // all loads in flight at once (indices clamped into the array); pairs
+        // past its end read as the unused key
+        uint32_t kk[kSegPer];
+        float vv[kSegPer];
 #pragma unroll
-        for (int k = 0; k < kSegPer; ++k) {
-            const uint64_t i = base + 64 * k + lane;
-            sk[w][64 * k + lane] = i < n ? keys[i] : kAdjointNoKey;
-            sv[w][64 * k + lane] = i < n ? vals[i] : 0.0f;
+        for (int i = 0; i < kSegPer; ++i) {
+            const uint64_t j = min(mine + i, (uint64_t)n - 1);
+            kk[i] = keys[j];
+            vv[i] = vals[j];
         }
-        // the key before the tile: a run that starts there is summed by the
-        // lane of the tile it starts in
+        const uint64_t after = min(base + kSegTile + lane, (uint64_t)n - 1);
+        const uint32_t ak = keys[after];
+        const float av = vals[after];
         const uint32_t kprev = base > 0 ? keys[base - 1] : kAdjointNoKey;
+#pragma unroll
+        for (int i = 0; i < kSegPer; ++i)
+            if (mine + i >= n) kk[i] = kAdjointNoKey;
+        nk[w][lane] = base + kSegTile + lane < n ? ak : kAdjointNoKey;
+        nv[w][lane] = av;
+
+        // the key before this lane's pairs and the one after them (for the
+        // last lane, the first pair after the tile: lane 0's `ak`)
+        const uint32_t before = wave_shr1_or(kk[kSegPer - 1], kprev);
+        const uint32_t ak0 = __builtin_amdgcn_readfirstlane(ak);
+        const uint32_t next = wave_shl1_or(kk[0], base + kSegTile < n ? ak0 : kAdjointNoKey);
+        // walk: runs that start here are summed; `head` while the pairs still
+        // continue the run of `before`
+        uint32_t run = before, nhead = kSegPer;
+        bool head = true;
+        float sum = 0.0f;
+#pragma unroll
+        for (int i = 0; i < kSegPer; ++i) {
+            if (kk[i] != run) {
+                if (!head && run != kAdjointNoKey) grad[run] = sum;
+                if (head) nhead = i;
+                head = false;
+                run = kk[i];
+                sum = 0.0f;
+            }
+            if (!head) sum += vv[i];
+        }
+        const bool goes_on = run == next;
+        if (!head && !goes_on && run != kAdjointNoKey) grad[run] = sum;
+        // carries: `out` is the running sum of the run this lane ends with,
+        // `known` once it is, `owned` when that run started in this tile
+        float out = sum;
+        bool known = !head;
+        bool owned = !head;
+        while (__ballot(!known) != 0ull) {
+            const float cin = __builtin_bit_cast(float, wave_shr1_or(__builtin_bit_cast(uint32_t, out), 0u));
+            const bool kin = wave_shr1_or((uint32_t)known, 1u) != 0u;
+            const bool oin = wave_shr1_or((uint32_t)owned, 0u) != 0u;
+            if (!known && kin) {
+                float r = cin;
+#pragma unroll
+                for (int i = 0; i < kSegPer; ++i) r += vv[i];
+                out = r;
+                owned = oin;
+                known = true;
+            }
+        }
+        const float cin = __builtin_bit_cast(float, wave_shr1_or(__builtin_bit_cast(uint32_t, out), 0u));
+        const bool oin = wave_shr1_or((uint32_t)owned, 0u) != 0u;
+        // the run arriving from the lane before ends in this lane's head
+        if (oin && before != kAdjointNoKey && nhead > 0 && nhead < kSegPer) {
+            float r = cin;
+#pragma unroll
+            for (int i = 0; i < kSegPer; ++i)
+                if (i < (int)nhead) r += vv[i];
+            grad[before] = r;
+        }
+        // ... or exactly at its last pair
+        if (head && owned && !goes_on && run != kAdjointNoKey) grad[run] = out;
+        // the run going on past the tile (last lane)
         wave_lds_sync();
-        for (int e = kSegPer * lane; e < kSegPer * (lane + 1); ++e) {
-            const uint32_t k = sk[w][e];
-            if (k == kAdjointNoKey || k == (e > 0 ? sk[w][e - 1] : kprev)) continue;
-            float sum = 0.0f;
-            int j = e;
-            for (; j < kSegTile && sk[w][j] == k; ++j) sum += sv[w][j];
-            if (j == kSegTile)
-                for (uint64_t g = base + kSegTile; g < n && keys[g] == k; ++g) sum += vals[g];
-            grad[k] = sum;
+        if (lane == 63 && goes_on && owned && run != kAdjointNoKey) {
+            float r = out;
+            int j = 0;
+            for (; j < 64 && nk[w][j] == run; ++j) r += nv[w][j];
+            if (j == 64)
+                for (uint64_t g = base + kSegTile + 64; g < n && keys[g] == run; ++g) r += vals[g];
+            grad[run] = r;
         }
         wave_lds_sync();
     }
