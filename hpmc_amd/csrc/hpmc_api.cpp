@@ -90,7 +90,7 @@ extern "C" __global__ __launch_bounds__(256) void hpmc_fill_f32(float* p, uint64
 
 namespace hpmc_host {
 
-// Scratch of the adjoint, grown on demand: 8 (key, value) pairs per vertex
+// Scratch of the adjoint, grown on demand: 7 (key, value) pairs per vertex
 // twice (sort input and output), per-vertex dL/dT terms, partial sums and
 // the radix sort's temporary storage.
 struct AdjointScratch {
@@ -132,7 +132,7 @@ inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, const Kernel& k, fl
                            const float* gout, float* grad_field, float* grad_iso, const uint64_t* cases,
                            hipStream_t stream, std::string* err)
 {
-    const uint64_t npairs = 8ull * nvert;
+    const uint64_t npairs = (uint64_t)hpmc::kAdjointPairs * nvert;
     if (npairs > 0xffffffffull) {
         *err = "too many vertices for the adjoint";
         return HPMC_ERR_SIZE;
@@ -140,12 +140,12 @@ inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, const Kernel& k, fl
     if (npairs > s.cap_pairs || !s.partials) {
         ADJ_TRY(hipStreamSynchronize(stream));
         adjoint_free(s);
-        const uint64_t cap = std::max<uint64_t>(npairs, 8);
+        const uint64_t cap = std::max<uint64_t>(npairs, hpmc::kAdjointPairs);
         for (int i = 0; i < 2; ++i) {
             ADJ_TRY(hipMalloc(&s.keys[i], cap * sizeof(uint32_t)));
             ADJ_TRY(hipMalloc(&s.vals[i], cap * sizeof(float)));
         }
-        ADJ_TRY(hipMalloc(&s.dT, cap / 8 * sizeof(float)));
+        ADJ_TRY(hipMalloc(&s.dT, cap / hpmc::kAdjointPairs * sizeof(float)));
         ADJ_TRY(hipMalloc(&s.partials, hpmc::kReduceBlocks * sizeof(float)));
         ADJ_TRY(rocprim::radix_sort_pairs(nullptr, s.sort_tmp_bytes, s.keys[0], s.keys[1], s.vals[0], s.vals[1],
                                           (unsigned)cap, 0, 32, stream));
@@ -174,8 +174,7 @@ inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, const Kernel& k, fl
         void* args[] = {&vols, &Ta, &it, &lo, &tt, &cnt, &cs, &go, &ko, &vo, &dt};
         ADJ_TRY(launch(k, (unsigned)p.emit_wgs, 256, args, stream));
     }
-    // keys < n_lattice < 2^end_bit - 1, so the unused-pair key (all ones)
-    // sorts last on the low end_bit bits as well
+    // keys < n_lattice <= 2^end_bit - 1: the sort looks at end_bit bits
     int end_bit = 1;
     while (end_bit < 32 && ((uint64_t)1 << end_bit) <= p.n_lattice) ++end_bit;
     size_t tmp_bytes = s.sort_tmp_bytes;

@@ -879,9 +879,11 @@ enum EmitMode { kEmitContinuous = 0, kEmitBinary = 1, kEmitGradient = 2, kEmitAd
 
 // Adjoint of continuous emission: per output vertex o, dL/d(vertex o) is
 // read from gout[6o..6o+5] (normal, position) and the vertex's
-// contributions to dL/dfield are written as 8 (key, value) pairs at
-// keys/vals[8o..8o+7] (key = lattice_base + sample index, kAdjointNoKey for
-// an unused pair) and its contribution to dL/dT at dT[o].
+// contributions to dL/dfield are written as kAdjointPairs (key, value)
+// pairs at keys/vals[7o..7o+6] (key = lattice_base + sample index) and its
+// contribution to dL/dT at dT[o].  kAdjointNoKey, above every sample index,
+// pads the run-sum kernel's tiles.
+constexpr int kAdjointPairs = 7;
 constexpr uint32_t kAdjointNoKey = 0xffffffffu;
 struct AdjointIO {
     const float* gout;
@@ -1295,8 +1297,10 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                 const int Ax = cx0 + s.ax, Ay = cy0 + s.ay, Az = cz + s.az;
                 const int Bx = Ax + (axis == 0), By = Ay + (axis == 1), Bz = Az + (axis == 2);
                 const uint32_t lb = dc.lattice_base;
-                uint4 k0, k1;
-                float4 w0, w1;
+                uint4 k0;
+                uint3 k1;
+                float4 w0;
+                float3 w1;
                 k0.x = lb + field.index(Ax, Ay, Az);
                 w0.x = dva;
                 k0.y = lb + field.index(Bx, By, Bz);
@@ -1313,14 +1317,23 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                 w1.y = -wy * tt;
                 k1.z = lb + field.index(Bx, By, Bz + 1);
                 w1.z = -wz * tt;
-                k1.w = kAdjointNoKey;
-                w1.w = 0.0f;
-                uint4* kd = (uint4*)(adj.keys + (size_t)o * 8);
-                float4* vd = (float4*)(adj.vals + (size_t)o * 8);
-                kd[0] = k0;
-                kd[1] = k1;
-                vd[0] = w0;
-                vd[1] = w1;
+                // seven pairs per vertex: 4-byte aligned dword stores
+                uint32_t* kd = adj.keys + (size_t)o * kAdjointPairs;
+                float* vd = adj.vals + (size_t)o * kAdjointPairs;
+                kd[0] = k0.x;
+                kd[1] = k0.y;
+                kd[2] = k0.z;
+                kd[3] = k0.w;
+                kd[4] = k1.x;
+                kd[5] = k1.y;
+                kd[6] = k1.z;
+                vd[0] = w0.x;
+                vd[1] = w0.y;
+                vd[2] = w0.z;
+                vd[3] = w0.w;
+                vd[4] = w1.x;
+                vd[5] = w1.y;
+                vd[6] = w1.z;
                 adj.dT[o] = dT;
             }
         } else if constexpr (CB::kActive) {
