@@ -128,6 +128,16 @@ inline void adjoint_free(AdjointScratch& s)
 // of the last extraction of plan p (nvert vertices at threshold T), from
 // dL/dvertices gout (nvert x 6, normal then position).  Deterministic:
 // per-vertex contributions, a stable sort by sample, in-order run sums.
+// rocPRIM's onesweep radix sort with 9-bit digits, for key widths where that
+// needs one pass fewer than the 8-bit digits of its default configuration
+// for 4-byte keys and values on gfx950 (1024-thread blocks, 8 items per
+// thread, match ranking): 17-18 and 25-27 bits (81M pairs on 26 bits: 2.4 ->
+// 1.55 ms).  A stable sort has one result, so the choice changes no bits.
+using SortCfg9 = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 8>, rocprim::kernel_config<1024, 8>, 9,
+                                        rocprim::block_radix_rank_algorithm::match>>;
+
 inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, const Kernel& k, float T, uint32_t nvert,
                            const float* gout, float* grad_field, float* grad_iso, const uint64_t* cases,
                            hipStream_t stream, std::string* err)
@@ -147,8 +157,12 @@ inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, const Kernel& k, fl
         }
         ADJ_TRY(hipMalloc(&s.dT, cap / hpmc::kAdjointPairs * sizeof(float)));
         ADJ_TRY(hipMalloc(&s.partials, hpmc::kReduceBlocks * sizeof(float)));
-        ADJ_TRY(rocprim::radix_sort_pairs(nullptr, s.sort_tmp_bytes, s.keys[0], s.keys[1], s.vals[0], s.vals[1],
+        size_t bytes8 = 0, bytes9 = 0;
+        ADJ_TRY(rocprim::radix_sort_pairs(nullptr, bytes8, s.keys[0], s.keys[1], s.vals[0], s.vals[1],
                                           (unsigned)cap, 0, 32, stream));
+        ADJ_TRY(rocprim::radix_sort_pairs<SortCfg9>(nullptr, bytes9, s.keys[0], s.keys[1], s.vals[0], s.vals[1],
+                                                    (unsigned)cap, 0, 32, stream));
+        s.sort_tmp_bytes = std::max(bytes8, bytes9);
         ADJ_TRY(hipMalloc(&s.sort_tmp, std::max<size_t>(s.sort_tmp_bytes, 16)));
         s.cap_pairs = cap;
     }
@@ -178,8 +192,13 @@ inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, const Kernel& k, fl
     int end_bit = 1;
     while (end_bit < 32 && ((uint64_t)1 << end_bit) <= p.n_lattice) ++end_bit;
     size_t tmp_bytes = s.sort_tmp_bytes;
-    ADJ_TRY(rocprim::radix_sort_pairs(s.sort_tmp, tmp_bytes, s.keys[0], s.keys[1], s.vals[0], s.vals[1],
-                                      (unsigned)npairs, 0, end_bit, stream));
+    if ((end_bit + 8) / 9 < (end_bit + 7) / 8) {
+        ADJ_TRY(rocprim::radix_sort_pairs<SortCfg9>(s.sort_tmp, tmp_bytes, s.keys[0], s.keys[1], s.vals[0],
+                                                    s.vals[1], (unsigned)npairs, 0, end_bit, stream));
+    } else {
+        ADJ_TRY(rocprim::radix_sort_pairs(s.sort_tmp, tmp_bytes, s.keys[0], s.keys[1], s.vals[0], s.vals[1],
+                                          (unsigned)npairs, 0, end_bit, stream));
+    }
     hipLaunchKernelGGL(hpmc_adjoint_segments, dim3(2048), dim3(256), 0, stream, (const uint32_t*)s.keys[1],
                        (const float*)s.vals[1], (uint32_t)npairs, grad_field);
     hipLaunchKernelGGL(hpmc_sum_partials, dim3(hpmc::kReduceBlocks), dim3(256), 0, stream, (const float*)s.dT,

@@ -41,7 +41,9 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-@pytest.mark.parametrize("shape,ext", [((40, 40, 40), (1.0, 1.0, 1.0)), ((37, 52, 29), (2.0, 0.5, 1.0))])
+# 48^3 = 110,592 samples: 17 key bits, sorted with 9-bit digits (two passes)
+@pytest.mark.parametrize("shape,ext", [((40, 40, 40), (1.0, 1.0, 1.0)), ((37, 52, 29), (2.0, 0.5, 1.0)),
+                                       ((48, 48, 48), (1.0, 1.0, 1.0))])
 def test_adjoint_matches_fp64_oracle(shape, ext):
     f = sphere_noise(shape, 7)
     T = 0.0
