@@ -1364,10 +1364,11 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
 // Adjoint reduction, after the (key, value) pairs are sorted by key with
 // ties in vertex order: every run of equal keys is summed from 0 in array
 // order, one addition after the other, so the result does not depend on the
-// launch.  A wave takes a tile of kSegTile pairs, 16 consecutive ones per
-// lane in registers.  Each lane walks its 16 pairs once: runs that start
-// and end in them are written at once; the run its pairs end with, if it
-// goes on into the next lane, leaves its running sum there as a carry.  A
+// launch.  A wave takes a tile of kSegTile pairs, loaded coalesced into LDS
+// and read back 16 consecutive ones per lane.  Each lane walks its 16 pairs
+// once: runs that start and end in them are written at once; the run its
+// pairs end with, if it goes on into the next lane, leaves its running sum
+// there as a carry.  A
 // lane whose 16 pairs all continue the run of the lane before passes the
 // carry on after adding its own (a few rounds at most: a sample feeds at
 // most a few dozen vertices), and the lane where the run ends adds its head
@@ -1391,39 +1392,48 @@ __device__ __forceinline__ void adjoint_segments_body(const uint32_t* __restrict
                                                       const float* __restrict__ vals, uint32_t n,
                                                       float* __restrict__ grad)
 {
-    __shared__ uint32_t nk[4][64];   // per wave: the 64 pairs after its tile
-    __shared__ float nv[4][64];
+    // per wave: its tile and the 64 pairs after it, one pad word per 16
+    // pairs so that the lanes reading 16 consecutive pairs hit different banks
+    constexpr int kStaged = kSegTile + 64;
+    __shared__ uint32_t sk[4][kStaged + kStaged / 16];
+    __shared__ float sv[4][kStaged + kStaged / 16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
     const uint64_t ntiles = ((uint64_t)n + kSegTile - 1) / kSegTile;
     for (uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + w; t < ntiles; t += nwaves) {
         const uint64_t base = t * kSegTile;
-        const uint64_t mine = base + (uint64_t)kSegPer * lane;
-        // all loads in flight at once (indices clamped into the array); pairs
-        // past its end read as the unused key
+        // coalesced loads, all in flight at once (indices clamped into the
+        // array; pairs past its end read as the unused key), staged in LDS
+        constexpr int kLoads = kStaged / 64;
+        uint32_t lk[kLoads];
+        float lv[kLoads];
+#pragma unroll
+        for (int k = 0; k < kLoads; ++k) {
+            const uint64_t i = min(base + 64 * k + lane, (uint64_t)n - 1);
+            lk[k] = keys[i];
+            lv[k] = vals[i];
+        }
+        const uint32_t kprev = base > 0 ? keys[base - 1] : kAdjointNoKey;
+#pragma unroll
+        for (int k = 0; k < kLoads; ++k) {
+            const int e = 64 * k + lane;
+            sk[w][e + (e >> 4)] = base + e < n ? lk[k] : kAdjointNoKey;
+            sv[w][e + (e >> 4)] = lv[k];
+        }
+        wave_lds_sync();
         uint32_t kk[kSegPer];
         float vv[kSegPer];
 #pragma unroll
         for (int i = 0; i < kSegPer; ++i) {
-            const uint64_t j = min(mine + i, (uint64_t)n - 1);
-            kk[i] = keys[j];
-            vv[i] = vals[j];
+            kk[i] = sk[w][17 * lane + i];   // pair 16 lane + i
+            vv[i] = sv[w][17 * lane + i];
         }
-        const uint64_t after = min(base + kSegTile + lane, (uint64_t)n - 1);
-        const uint32_t ak = keys[after];
-        const float av = vals[after];
-        const uint32_t kprev = base > 0 ? keys[base - 1] : kAdjointNoKey;
-#pragma unroll
-        for (int i = 0; i < kSegPer; ++i)
-            if (mine + i >= n) kk[i] = kAdjointNoKey;
-        nk[w][lane] = base + kSegTile + lane < n ? ak : kAdjointNoKey;
-        nv[w][lane] = av;
+        const uint32_t ak0 = sk[w][kSegTile + (kSegTile >> 4)];   // the first pair after the tile
 
         // the key before this lane's pairs and the one after them (for the
-        // last lane, the first pair after the tile: lane 0's `ak`)
+        // last lane, the first staged pair after the tile)
         const uint32_t before = wave_shr1_or(kk[kSegPer - 1], kprev);
-        const uint32_t ak0 = __builtin_amdgcn_readfirstlane(ak);
-        const uint32_t next = wave_shl1_or(kk[0], base + kSegTile < n ? ak0 : kAdjointNoKey);
+        const uint32_t next = wave_shl1_or(kk[0], ak0);
         // walk: runs that start here are summed; `head` while the pairs still
         // continue the run of `before`
         uint32_t run = before, nhead = kSegPer;
@@ -1473,11 +1483,11 @@ __device__ __forceinline__ void adjoint_segments_body(const uint32_t* __restrict
         // ... or exactly at its last pair
         if (head && owned && !goes_on && run != kAdjointNoKey) grad[run] = out;
         // the run going on past the tile (last lane)
-        wave_lds_sync();
         if (lane == 63 && goes_on && owned && run != kAdjointNoKey) {
             float r = out;
             int j = 0;
-            for (; j < 64 && nk[w][j] == run; ++j) r += nv[w][j];
+            for (; j < 64 && sk[w][kSegTile + j + ((kSegTile + j) >> 4)] == run; ++j)
+                r += sv[w][kSegTile + j + ((kSegTile + j) >> 4)];
             if (j == 64)
                 for (uint64_t g = base + kSegTile + 64; g < n && keys[g] == run; ++g) r += vals[g];
             grad[run] = r;
