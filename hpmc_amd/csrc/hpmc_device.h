@@ -1499,7 +1499,7 @@ __device__ __forceinline__ void adjoint_segments_body(const uint32_t* __restrict
 // dL/dT: a fixed-shape reduction of the per-vertex terms (partials of
 // kReduceBlocks contiguous ranges, then one block over the partials), the
 // same for every launch.
-constexpr int kReduceBlocks = 256;
+constexpr int kReduceBlocks = 2048;
 __device__ __forceinline__ float block_sum_f(float v, float* sh)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
