@@ -1368,16 +1368,18 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
 // and read back 16 consecutive ones per lane.  Each lane walks its 16 pairs
 // once: runs that start and end in them are written at once; the run its
 // pairs end with, if it goes on into the next lane, leaves its running sum
-// there as a carry.  A
-// lane whose 16 pairs all continue the run of the lane before passes the
-// carry on after adding its own (a few rounds at most: a sample feeds at
-// most a few dozen vertices), and the lane where the run ends adds its head
-// pairs to the carry and writes the sum.  Runs arriving from the previous
-// tile belong to it; a run going on past the tile is finished from the
-// next 64 pairs, staged in LDS (and from global memory in the unlikely case
-// that it is longer still).  Launched with 256 threads per workgroup.
+// there as a carry.  A lane whose 16 pairs all continue the run of the lane
+// before passes the carry on after adding its own (a few rounds at most: a
+// sample feeds at most a few dozen vertices), and the lane where the run
+// ends adds its head pairs to the carry and writes the sum.  Runs arriving
+// from the previous tile belong to it; a run going on past the tile is
+// finished from the next 64 pairs, staged in LDS (and from global memory in
+// the unlikely case that it is longer still).  Launched with 256 threads per
+// workgroup.
 constexpr int kSegPer = 16;
 constexpr int kSegTile = 64 * kSegPer;
+// x of lane - 1 (lane0 in lane 0) and of lane + 1 (lane63 in lane 63): DPP
+// wave_shr:1 / wave_shl:1
 __device__ __forceinline__ uint32_t wave_shr1_or(uint32_t x, uint32_t lane0)
 {
     const uint32_t y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x138, 0xf, 0xf, false);   // wave_shr:1
