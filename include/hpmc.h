@@ -257,7 +257,7 @@ int32_t HPMCgetLayout(struct HPMCHistoPyramid* h, int32_t out[5]);
 /* ---- batched builds (no counterpart in the reference) -------------------
  *
  * Many FP32 volumes of uneven sizes built and extracted together: one
- * classification launch, one two-pass scan and one persistent extraction
+ * classification launch, one scan launch and one persistent extraction
  * launch cover every volume, and the chain can be captured once and replayed
  * as a hipGraph.  Each volume's count and vertex list are those of a
  * HistoPyramid of that volume alone (same codes, same order, same bits); the
