@@ -130,8 +130,15 @@ def test_autograd_c4_loss_against_fp64_and_finite_differences():
         assert abs(got[idx] - est) <= 1e-4 * np.abs(ref_f).max(), (idx, got[idx], est)
 
 
-def test_batch_adjoint_equals_single_volume_adjoints():
-    shapes = [(33, 40, 28), (48, 48, 48), (20, 64, 36), (57, 31, 44)]
+# Batch and single builds sort different key widths, with different digit
+# sizes: the first set's batch has 19 key bits (8-bit digits) and its 17-bit
+# volumes alone use 9-bit digits; the second set's batch holds 2^24+ samples
+# (25 key bits, 9-bit digits) and its volumes alone 22-23 bits (8-bit
+# digits).  Either way every sample gets the same contributions in the same
+# order.
+@pytest.mark.parametrize("shapes", [[(33, 40, 28), (48, 48, 48), (20, 64, 36), (57, 31, 44)],
+                                    [(160, 160, 160), (200, 150, 180), (128, 256, 130), (150, 170, 160)]])
+def test_batch_adjoint_equals_single_volume_adjoints(shapes):
     fields = [sphere_noise(s, 30 + i) for i, s in enumerate(shapes)]
     vols = [torch.from_numpy(f).cuda() for f in fields]
     b = Batch()
