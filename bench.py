@@ -531,22 +531,34 @@ def main():
                      "traffic": traffic() if n == 512 else None, "algorithmic_bytes": lattice_bytes},
     }
     iso.close()
-    if not args.no_pipeline:
+    def extra(key, fn):
+        # a failing side measurement is reported in its object; the line
+        # with the headline is printed whatever happens to the others
+        try:
+            line[key] = fn()
+        except Exception as exc:  # noqa: BLE001 - reported, not hidden
+            line[key] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+            print("bench: %s failed: %s" % (key, exc), file=sys.stderr, flush=True)
+
+    def pipelined():
         pl = measure_pipelined(n, args.steps, args.warmup)
         pl["hbm_frac"] = algo_bytes / (pl["ms_per_frame"] * 1e-3) / (HBM_PEAK_GBS * 1e9)
-        line["pipelined"] = pl
+        return pl
+
+    if not args.no_pipeline:
+        extra("pipelined", pipelined)
     if not args.no_unorm8:
-        line["unorm8"] = measure_unorm8(n, args.steps, args.warmup)
+        extra("unorm8", lambda: measure_unorm8(n, args.steps, args.warmup))
     if not args.no_snippet:
-        line["snippet"] = measure_snippet(n, args.steps, args.warmup)
+        extra("snippet", lambda: measure_snippet(n, args.steps, args.warmup))
     if not args.no_texture256:
-        line["texture256"] = measure_texture256(args.steps, args.warmup)
+        extra("texture256", lambda: measure_texture256(args.steps, args.warmup))
     if not args.no_batch:
-        line["batched"] = measure_batched(args.batch_reps)
+        extra("batched", lambda: measure_batched(args.batch_reps))
     if not args.no_adjoint:
-        line["adjoint"] = measure_adjoint(args.batch_reps)
+        extra("adjoint", lambda: measure_adjoint(args.batch_reps))
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(n, args.cpu_reps)
+        extra("cpu_baseline", lambda: cpu_baseline(n, args.cpu_reps))
     print(json.dumps(line))
 
 
