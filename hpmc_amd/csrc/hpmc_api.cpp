@@ -34,7 +34,10 @@
 #define HPMC_VOLUME_FIELD (hpmc::VolumeField{(const float*)dd.field, dd.g.sx, dd.g.sy, dd.g.sz})
 #define HPMC_VOLUME4_FIELD (hpmc::Volume4Field{(const float4*)dd.field, dd.g.sx, dd.g.sy, dd.g.sz})
 #define HPMC_VOLUME8_FIELD (hpmc::Volume8Field{(const uint8_t*)dd.field, dd.g.sx, dd.g.sy, dd.g.sz})
+#define HPMC_VOLUME_STREAM_FIELD                                                                     \
+    (hpmc::VolumeStreamField{{(const float*)dd.field, dd.g.sx, dd.g.sy, dd.g.sz}})
 HPMC_DEFINE_BUILD_KERNELS(volume, HPMC_VOLUME_FIELD)
+HPMC_DEFINE_BUILD_KERNELS(volume_stream, HPMC_VOLUME_STREAM_FIELD)
 HPMC_DEFINE_BUILD_KERNELS(volume4, HPMC_VOLUME4_FIELD)
 HPMC_DEFINE_BUILD_KERNELS(volume8, HPMC_VOLUME8_FIELD)
 HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume, HPMC_VOLUME_FIELD, hpmc::kEmitContinuous, false)
@@ -599,7 +602,11 @@ bool setup(HPMCHistoPyramid* h)
             h->k_emit.builtin = h->binary_active ? (const void*)hpmc_emit_volume4_binary
                                                  : (const void*)hpmc_emit_volume4_gradient;
         } else {
-            h->k_classify.builtin = (const void*)hpmc_classify_volume;
+            // lattices past the 256 MiB MALL stream through classification
+            // with non-temporal loads (VolumeStreamField)
+            const uint64_t bytes = 4ull * (uint64_t)h->size[0] * (uint64_t)h->size[1] * (uint64_t)h->size[2];
+            h->k_classify.builtin = bytes >= (256ull << 20) ? (const void*)hpmc_classify_volume_stream
+                                                            : (const void*)hpmc_classify_volume;
             h->k_codes.builtin = (const void*)hpmc_codes_volume;
             h->k_emit.builtin = h->binary_active ? (const void*)hpmc_emit_volume_binary
                                                  : (const void*)hpmc_emit_volume;

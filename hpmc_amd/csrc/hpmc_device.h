@@ -223,6 +223,9 @@ __device__ __forceinline__ uint32_t wave_shr1_u32(uint32_t x)
 // ---------------------------------------------------------------------------
 // Field sources.  at(i,j,k) returns the sample at lattice index (i,j,k).
 
+// The NT bit of a buffer load's cache policy (VolumeStreamField).
+constexpr int kStreamLoad = 2;
+
 // FP32 volume, x fastest; indices clamp to the lattice like a NEAREST/CLAMP
 // 3D texture (apps/texture3d/texture3d.cpp:135-139).
 //
@@ -275,6 +278,21 @@ struct VolumeField {
     }
     // the sample value emission interpolates (fetch returns it as is)
     __device__ __forceinline__ float sample(Plane p, uint32_t idx) const { return fetch(p, idx); }
+};
+
+// An FP32 volume whose classification row loads are non-temporal: for
+// lattices larger than the 256 MiB MALL, which stream through classification
+// once and cannot stay cached for emission anyway; their lines then leave
+// L2 and the MALL to other data (512^3: classification 92.5 -> 87 us,
+// emission 49 -> 52 us, 150 -> 146 us per build + extract; three frames in
+// flight 131 -> 122 us per frame).  Smaller volumes and batches keep
+// VolumeField: their emission finds the planes it re-reads in the MALL
+// (a 256^3 volume measured 1.3% and the batched build 3.5% slower streamed).
+struct VolumeStreamField : VolumeField {
+    __device__ __forceinline__ float fetch2(Plane p, uint32_t col, uint32_t row) const
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p, col, row, kStreamLoad));
+    }
 };
 
 // 8-bit normalised volume, the texture3d demo's field (a GL_ALPHA8 texture
