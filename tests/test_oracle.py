@@ -138,6 +138,27 @@ def test_level4_sums():
     assert runs[-1] == len(cell)
 
 
+def test_level4_morton_keys_hand_worked():
+    # 33x17x6 lattice: cells 32x16x5, tiles of 16x8 texels (32x16 cells of
+    # the doubled image), two tiles per row: slice z starts at doubled
+    # (32 (z mod 2), 16 (z div 2)).  The 64x64 doubled image has 4x4 blocks
+    # of 16x16 cells; block (X, Y) has Morton key x-bit-low: (0,0)=0,
+    # (1,0)=1, (0,1)=2, (1,1)=3, (2,0)=4, (3,0)=5, (2,1)=6, (3,1)=7,
+    # (0,2)=8, (1,2)=9.
+    cells = (32, 16, 5)
+    lay = oracle.layout((33, 17, 6), cells)
+    assert lay == (16, 8, 2, 4, 5)
+    counts = np.zeros((5, 16, 32), np.uint8)
+    counts[:, 3, 20] = np.arange(1, 6)      # cell x = 20 lies in block column 1 of its tile
+    counts[4, 9, 2] = 7                     # slice 4, block column 0
+    l4 = oracle.level4(counts, cells, lay)
+    expect = np.zeros(16, np.uint32)
+    # slice: 0 -> block (1,0), 1 -> (3,0), 2 -> (1,1), 3 -> (3,1), 4 -> (1,2) and (0,2)
+    expect[[1, 5, 3, 7, 9]] = [1, 2, 3, 4, 5]
+    expect[8] = 7
+    assert l4.tolist() == expect.tolist()
+
+
 # ------------------------------------------------------------------ emit ---
 def _list(f, T, ext=(1.0, 1.0, 1.0)):
     shape = f.shape[::-1]
@@ -163,6 +184,49 @@ def test_emit_linear_field_exact():
     py = out[:, 4] * cells[1]
     vals = px + 0.25 * py
     np.testing.assert_allclose(vals, 3.3, atol=1e-6)
+
+
+def _corners(codes, cell, slot, cells):
+    """First corner a (lattice index) and axis of each listed vertex."""
+    flat = np.asarray(codes).reshape(-1)
+    s = oracle.CASE_SLOTS[flat[cell], slot]
+    x = cell % cells[0]
+    y = (cell // cells[0]) % cells[1]
+    z = cell // (cells[0] * cells[1])
+    a = np.stack([x + s[:, 0], y + s[:, 1], z + s[:, 2]], axis=1).astype(np.int64)
+    return a, s[:, 3]
+
+
+def test_emit_linear_field_normals_closed_form():
+    # f = 0.5 x - 0.25 y + 0.125 z (lattice units, exact in FP32), T = 1.3,
+    # extent (1, 2, 3).  Forward samples of a linear field are f(a) + g_q and
+    # f(b) + g_q, so mix(na, nb, t) = mix(va, vb, t) + g = T + g and the
+    # normal is (T - mix(na, nb, t)) ext/cells = -g ext/cells exactly, for
+    # vertices whose six forward samples lie inside the lattice.  Dropping
+    # the T offset or inverting the per-axis scale fails this.
+    X, Y, Z = 11, 9, 8
+    T, ext = 1.3, (1.0, 2.0, 3.0)
+    g = np.array([0.5, -0.25, 0.125])
+    k = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    f = (g[0] * k[2] + g[1] * k[1] + g[2] * k[0]).astype(np.float32)
+    codes, cell, slot, cells = _list(f, T, ext=ext)
+    a, axis = _corners(codes, cell, slot, cells)
+    b = a + np.eye(3, dtype=np.int64)[axis]
+    interior = ((b + 1) <= np.array([X - 1, Y - 1, Z - 1])).all(axis=1)
+    assert interior.sum() > 100
+    sc = np.array(ext) / np.array(cells)
+    want = -g * sc
+    out, t = oracle.emit(f, codes, cell, slot, T, ext=ext)
+    np.testing.assert_allclose(out[interior, :3], np.broadcast_to(want, (interior.sum(), 3)), rtol=0, atol=1e-12)
+    # positions: a + t axis, scaled; t exact from the linear field
+    pos = (a + t[:, None] * np.eye(3)[axis]) * sc
+    np.testing.assert_allclose(out[:, 3:], pos, rtol=0, atol=1e-12)
+    val = (out[:, 3:] / sc) @ g
+    np.testing.assert_allclose(val, float(np.float32(T)), rtol=0, atol=1e-12)   # T goes in as FP32
+    # FP32 emission: the same within FP32 rounding of terms of size |f| <= 6
+    o32 = oracle.emit(f, codes, cell, slot, T, ext=ext, fp64=False)
+    np.testing.assert_allclose(o32[interior, :3], np.broadcast_to(want, (interior.sum(), 3)), rtol=0,
+                               atol=8 * 6.0 * 2.0 ** -24 * sc.max())
 
 
 def test_emit_sphere_distance():
