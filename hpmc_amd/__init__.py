@@ -180,6 +180,23 @@ def _libc_free(p):
     libc.free(p)
 
 
+def _vertex_grads(grad_vertices, count, device):
+    """dL/dvertices for an adjoint call: a contiguous float32 tensor of
+    shape (count, 6) on the field's device (the kernel reads count * 6
+    floats), or None when there are no vertices."""
+    import torch
+    if grad_vertices is None:
+        if count:
+            raise ValueError("grad_vertices is None but the surface has %d vertices" % count)
+        return None
+    if grad_vertices.dtype != torch.float32 or grad_vertices.device != device:
+        raise ValueError("grad_vertices must be float32 on %s, got %s on %s"
+                         % (device, grad_vertices.dtype, grad_vertices.device))
+    if tuple(grad_vertices.shape) != (count, 6):
+        raise ValueError("grad_vertices must have shape (%d, 6), got %s" % (count, tuple(grad_vertices.shape)))
+    return grad_vertices.contiguous()
+
+
 def _stream_ptr(stream):
     return ctypes.c_void_p(stream.cuda_stream) if stream is not None else None
 
@@ -378,9 +395,9 @@ class IsoSurface:
         build + extraction from dL/dvertices (count, 6); deterministic."""
         import torch
         assert self._volume is not None, "adjoint needs a volume field"
+        gv = _vertex_grads(grad_vertices, self.count(), self._volume.device)
         gf = torch.empty_like(self._volume)
         gt = torch.empty((), dtype=torch.float32, device=self._volume.device)
-        gv = grad_vertices.contiguous() if grad_vertices is not None else None
         _check(lib().HPMCadjoint(self._traversal(), ctypes.c_void_p(gv.data_ptr()) if gv is not None else None,
                                  ctypes.c_void_p(gf.data_ptr()), ctypes.c_void_p(gt.data_ptr())), self.h)
         return gf, gt
@@ -499,10 +516,10 @@ class Batch:
         dL/dvertices (total, 6) of the concatenated output; deterministic."""
         import torch
         dev = self._volumes[0].device
+        gv = _vertex_grads(grad_vertices, self.counts()[2], dev)
         n_lat = sum(v.numel() for v in self._volumes)
         gf = torch.empty(n_lat, dtype=torch.float32, device=dev)
         gt = torch.empty((), dtype=torch.float32, device=dev)
-        gv = grad_vertices.contiguous() if grad_vertices is not None else None
         _check(lib().HPMCbatchAdjoint(self.b, ctypes.c_void_p(gv.data_ptr()) if gv is not None else None,
                                       ctypes.c_void_p(gf.data_ptr()), ctypes.c_void_p(gt.data_ptr())))
         grads = []

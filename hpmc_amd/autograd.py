@@ -9,6 +9,12 @@ backward: the library's adjoint (HPMCadjoint / HPMCbatchAdjoint): per-vertex
 
 The vertex count and topology are piecewise constant in the field, so the
 gradient is that of the vertex values at fixed topology.
+
+The fields are saved for backward, so an in-place change of a field between
+forward and backward raises torch's usual version-counter error instead of
+differentiating a surface the field no longer has.  The HistoPyramid (or
+batch) of the forward lives as long as the autograd graph, so backward may
+run more than once (retain_graph=True).
 """
 import torch
 
@@ -22,15 +28,18 @@ class _MarchingCubes(torch.autograd.Function):
         surf.set_volume(field.detach())
         verts, _ = surf.extract_all(float(iso))
         ctx.surf = surf
+        ctx.save_for_backward(field)
         return verts
 
     @staticmethod
+    @torch.autograd.function.once_differentiable
     def backward(ctx, grad_vertices):
+        (field,) = ctx.saved_tensors   # raises if the field changed in place
         surf = ctx.surf
-        gf, gt = surf.adjoint(grad_vertices)
-        surf.close()
-        ctx.surf = None
-        return gf, gt.reshape(()), None
+        if surf is None or surf.h is None:
+            raise RuntimeError("marching_cubes: the forward's surface is gone")
+        gf, gt = surf.adjoint(grad_vertices.contiguous())
+        return gf.view_as(field), gt.reshape(()), None
 
 
 def marching_cubes(field, iso, extent=(1.0, 1.0, 1.0)):
@@ -53,15 +62,18 @@ class _MarchingCubesBatch(torch.autograd.Function):
         out = torch.empty((max(total, 1), 6), dtype=torch.float32, device=fields[0].device)
         b.extract(out)
         ctx.batch = b
+        ctx.save_for_backward(*fields)
         counts_t = torch.from_numpy(counts.astype("int64"))
         ctx.mark_non_differentiable(counts_t)
         return out[:total], counts_t
 
     @staticmethod
+    @torch.autograd.function.once_differentiable
     def backward(ctx, grad_vertices, _grad_counts):
-        grads, gt = ctx.batch.adjoint(grad_vertices)
-        ctx.batch.close()
-        ctx.batch = None
+        _ = ctx.saved_tensors   # raises if a field changed in place
+        if ctx.batch is None or ctx.batch.b is None:
+            raise RuntimeError("marching_cubes_batch: the forward's batch is gone")
+        grads, gt = ctx.batch.adjoint(grad_vertices.contiguous())
         return (gt.reshape(()),) + tuple(grads)
 
 

@@ -317,9 +317,14 @@ struct HPMCHistoPyramid {
     // hipRTC module for snippets
     hipModule_t module = nullptr;
     uint32_t builder_id = 0;
-    // optional per-stage timing: event pairs recorded around each launch
+    // optional per-stage timing: event pairs recorded around each launch;
+    // finished pairs are folded into the totals once a stage holds
+    // kStagePairsMax of them, so profiling left on holds a bounded number
+    hipEvent_t ev_order = nullptr;   // HPMCsetStream: old stream -> new stream
     bool profiling = false;
     std::vector<hipEvent_t> stage_events[3];
+    double stage_ms[3] = {0.0, 0.0, 0.0};
+    uint32_t stage_n[3] = {0, 0, 0};
     // errors
     int32_t last_error = HPMC_OK;
     std::string last_msg;
@@ -638,18 +643,52 @@ bool setup(HPMCHistoPyramid* h)
 
 enum Stage { kStageClassify = 0, kStageScan = 1, kStageEmit = 2 };
 
+constexpr size_t kStagePairsMax = 1024;
+
+// Adds the elapsed times of a stage's recorded event pairs to its totals and
+// destroys the events (waits for the last pair).  False if an event failed.
+bool stage_fold(HPMCHistoPyramid* h, int st)
+{
+    std::vector<hipEvent_t>& ev = h->stage_events[st];
+    bool good = true;
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+        if (hipEventSynchronize(ev[i + 1]) != hipSuccess) {
+            good = false;
+            continue;
+        }
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) == hipSuccess) {
+            h->stage_ms[st] += t;
+            ++h->stage_n[st];
+        }
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    return good;
+}
+
 void stage_mark(HPMCHistoPyramid* h, int stage)
 {
     if (!h->profiling) return;
+    std::vector<hipEvent_t>& ev = h->stage_events[stage];
+    // an opening mark with the stage full: fold the finished pairs first
+    if (ev.size() % 2 == 0 && ev.size() >= 2 * kStagePairsMax) (void)stage_fold(h, stage);
     hipEvent_t e = nullptr;
     if (hipEventCreate(&e) != hipSuccess) return;
     (void)hipEventRecord(e, h->stream);
-    h->stage_events[stage].push_back(e);
+    ev.push_back(e);
 }
 
 bool launch_classify(HPMCHistoPyramid* h, float T)
 {
-    if (h->mode == kFieldVolume) HIP_TRY(h, hpmc_host::plan_set_field(h->plan, 0, volume_ptr(h), h->stream));
+    if (h->mode == kFieldVolume) {
+        // a volume reset to NULL after the configuration: no launch
+        if (!volume_ptr(h)) {
+            fail(h, HPMC_ERR_FIELD, "no field volume set");
+            return false;
+        }
+        HIP_TRY(h, hpmc_host::plan_set_field(h->plan, 0, volume_ptr(h), h->stream));
+    }
     stage_mark(h, kStageClassify);
     // 8-bit volumes classify raw bytes against the equivalent byte threshold
     const float Tc = h->mode == kFieldVolume && h->unorm8 ? unorm8_threshold(T) : T;
@@ -862,6 +901,9 @@ void HPMCdestroyHandle(struct HPMCHistoPyramid* h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     else (void)hipDeviceSynchronize();
     free_device(h);
+    if (h->ev_order) (void)hipEventDestroy(h->ev_order);
+    for (int st = 0; st < 3; ++st)
+        for (hipEvent_t e : h->stage_events[st]) (void)hipEventDestroy(e);
     delete h;
 }
 
@@ -1042,7 +1084,20 @@ const char* HPMCgetLastErrorString(struct HPMCHistoPyramid* h) { return h ? h->l
 int32_t HPMCsetStream(struct HPMCHistoPyramid* h, void* stream)
 {
     if (!h) return fail(nullptr, HPMC_ERR_NULL, "h == NULL");
-    h->stream = (hipStream_t)stream;
+    const hipStream_t next = (hipStream_t)stream;
+    // Work already queued on the old stream (a build whose extraction comes
+    // next) must finish before anything queued on the new one: the new
+    // stream waits for an event recorded on the old.  Without device state
+    // nothing has been queued.
+    if (next != h->stream && h->plan.d_vols) {
+        if (!h->ev_order && hipEventCreateWithFlags(&h->ev_order, hipEventDisableTiming) != hipSuccess) {
+            h->ev_order = nullptr;
+            return fail(h, HPMC_ERR_HIP, "failed to create the stream-order event");
+        }
+        if (hipEventRecord(h->ev_order, h->stream) != hipSuccess || hipStreamWaitEvent(next, h->ev_order, 0) != hipSuccess)
+            return fail(h, HPMC_ERR_HIP, "failed to order the new stream after the old one");
+    }
+    h->stream = next;
     ok(h);
     return HPMC_OK;
 }
@@ -1123,6 +1178,7 @@ int32_t HPMCgetCellCodes(struct HPMCHistoPyramid* h, uint8_t* codes)
     if (h->broken) return fail(h, HPMC_ERR_BROKEN, "h is broken");
     if (h->tainted && !setup(h)) return h->last_error;
     if (h->mode == kFieldVolume) {
+        if (!volume_ptr(h)) return fail(h, HPMC_ERR_FIELD, "no field volume set");
         if (hpmc_host::plan_set_field(h->plan, 0, volume_ptr(h), h->stream) != hipSuccess)
             return fail(h, HPMC_ERR_HIP, "descriptor update failed");
     }
@@ -1176,23 +1232,15 @@ int32_t HPMCsetProfiling(struct HPMCHistoPyramid* h, int32_t enable)
 int32_t HPMCgetStageTimes(struct HPMCHistoPyramid* h, double* ms, uint32_t* launches)
 {
     if (!h || !ms) return fail(h, HPMC_ERR_NULL, "NULL argument");
+    bool good = true;
     for (int st = 0; st < 3; ++st) {
-        std::vector<hipEvent_t>& ev = h->stage_events[st];
-        double total = 0.0;
-        uint32_t n = 0;
-        for (size_t i = 0; i + 1 < ev.size(); i += 2) {
-            if (hipEventSynchronize(ev[i + 1]) != hipSuccess) return fail(h, HPMC_ERR_HIP, "event sync failed");
-            float t = 0.0f;
-            if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) == hipSuccess) {
-                total += t;
-                ++n;
-            }
-        }
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-        ev.clear();
-        ms[st] = total;
-        if (launches) launches[st] = n;
+        good = stage_fold(h, st) && good;
+        ms[st] = h->stage_ms[st];
+        if (launches) launches[st] = h->stage_n[st];
+        h->stage_ms[st] = 0.0;
+        h->stage_n[st] = 0;
     }
+    if (!good) return fail(h, HPMC_ERR_HIP, "event sync failed");
     ok(h);
     return HPMC_OK;
 }
@@ -1264,6 +1312,7 @@ struct HPMCBatch {
     hpmc_host::AdjointScratch adjoint;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    float graph_threshold = 0.0f;    // the threshold the graph was captured at
 };
 
 namespace {
@@ -1480,6 +1529,7 @@ int32_t HPMCbatchCaptureGraph(struct HPMCBatch* b, float threshold, float* out, 
     }
     if (e != hipSuccess) return fail(nullptr, HPMC_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
     b->graph = g;
+    b->graph_threshold = threshold;
     BATCH_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
     ok(nullptr);
     return HPMC_OK;
@@ -1490,6 +1540,9 @@ int32_t HPMCbatchLaunchGraph(struct HPMCBatch* b)
     if (!b) return fail(nullptr, HPMC_ERR_NULL, "NULL batch");
     if (!b->graph_exec || b->tainted) return fail(nullptr, HPMC_ERR_ARG, "no captured graph for this configuration");
     BATCH_TRY(hipGraphLaunch(b->graph_exec, b->stream));
+    // the replayed build is at the captured threshold: later extractions
+    // and adjoints of this build use it
+    b->last_threshold = b->graph_threshold;
     b->built = true;
     b->counts_known = false;
     ok(nullptr);

@@ -216,29 +216,29 @@ struct DeviceShape {
         }                                                                                  \
     } while (0)
 
-// Lays out and allocates a plan for `specs`.  `classify_kernel` is the
-// kernel whose occupancy sizes the automatic zseg; `stack_bands` allows the
-// automatic shape to stack row bands of 4- and 8-wave rows (FP32 volumes).
-// Returns HPMC_OK or an error code with *err set (the plan is then empty).
-inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const Tuning& tune,
-                           const void* classify_kernel, const DeviceShape& dev, bool stack_bands,
-                           std::string* err)
+// Lays out a plan for specs[0 .. n) without touching device memory: the
+// layouts (lays[n]), launch shapes, per-volume descriptors (vols[n]) and the
+// sizes of every device array.  `classify_kernel` is the kernel whose
+// occupancy sizes the automatic zseg; `stack_bands` allows the automatic
+// shape to stack row bands of 4- and 8-wave rows (FP32 volumes).  Only an
+// error allocates (the message).  Returns HPMC_OK or an error code with
+// *err set.
+inline int32_t plan_describe(Plan& p, const VolumeSpec* specs, size_t n, Layout* lays, hpmc::VolDesc* vols,
+                             const Tuning& tune, const void* classify_kernel, const DeviceShape& dev,
+                             bool stack_bands, std::string* err)
 {
-    plan_free(p);
-    if (specs.empty() || specs.size() > 65535) {
+    if (n == 0 || n > 65535) {
         *err = "a build needs 1 to 65535 volumes";
         return HPMC_ERR_ARG;
     }
-    p.nvol = (int)specs.size();
-    p.layouts.resize(specs.size());
+    p.nvol = (int)n;
     int max_row_waves = 1;
-    for (size_t i = 0; i < specs.size(); ++i) {
-        const int32_t rc = layout_of(specs[i].size, specs[i].cells, &p.layouts[i], err);
+    for (size_t i = 0; i < n; ++i) {
+        const int32_t rc = layout_of(specs[i].size, specs[i].cells, &lays[i], err);
         if (rc != HPMC_OK) {
             char buf[32];
             snprintf(buf, sizeof buf, "volume %zu: ", i);
             *err = buf + *err;
-            p = Plan();
             return rc;
         }
         max_row_waves = std::max(max_row_waves, row_waves(specs[i].size[0]));
@@ -260,8 +260,8 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, classify_kernel, 64 * p.cls_waves, 0);
     // automatic zseg: fill the device once with all volumes' workgroups
     int64_t bands_cells = 0;
-    for (size_t i = 0; i < specs.size(); ++i) {
-        hpmc::Geom g0 = make_geom(specs[i], p.layouts[i], 8);
+    for (size_t i = 0; i < n; ++i) {
+        hpmc::Geom g0 = make_geom(specs[i], lays[i], 8);
         bands_cells += classify_bands(g0, p.cls_waves) * g0.cz;
     }
     const int64_t resident = (int64_t)(dev.cus > 0 ? dev.cus : 256) * (per_cu > 0 ? per_cu : 1);
@@ -269,16 +269,16 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
     const int zseg = tune.zseg > 0 ? tune.zseg : zseg_auto;
     if (zseg != 8 && zseg != 16 && zseg != 32) {
         *err = "zseg must be 8, 16 or 32";
-        p = Plan();
         return HPMC_ERR_ARG;
     }
 
-    std::vector<uint32_t> cls_vol, tile_vol_l4, tile_vol_col;
-    p.vols.resize(specs.size());
-    for (size_t i = 0; i < specs.size(); ++i) {
-        hpmc::VolDesc& d = p.vols[i];
+    p.n_wg = 0;
+    p.n_l4t = p.n_colt = 0;
+    p.n_cc = p.n_l4 = p.n_items_max = p.n_lattice = 0;
+    for (size_t i = 0; i < n; ++i) {
+        hpmc::VolDesc& d = vols[i];
         d = hpmc::VolDesc{};
-        d.g = make_geom(specs[i], p.layouts[i], zseg);
+        d.g = make_geom(specs[i], lays[i], zseg);
         d.field = specs[i].field;
         const uint64_t side = (uint64_t)1 << d.g.blocks_log2;
         d.n_level4 = (uint32_t)(side * side);
@@ -286,36 +286,29 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
         d.colcnt_off = p.n_cc;
         d.l4_base = (uint32_t)p.n_l4;
         d.cls_wg0 = p.n_wg;
-        d.l4_tile0 = (uint32_t)tile_vol_l4.size();
+        d.l4_tile0 = p.n_l4t;
         d.n_l4tiles = (d.n_level4 + hpmc::kScanTile - 1) / hpmc::kScanTile;
-        d.col_tile0 = (uint32_t)tile_vol_col.size();
+        d.col_tile0 = p.n_colt;
         d.n_coltiles = (d.n_cc + hpmc::kScanTile - 1) / hpmc::kScanTile;
         d.lattice_base = (uint32_t)p.n_lattice;
         const uint64_t wgs = (uint64_t)classify_bands(d.g, p.cls_waves) * d.g.npz;
         if (p.n_l4 + d.n_level4 > 0xffffffffull || p.n_wg + wgs > 0xffffffffull) {
             *err = "build too large for 32-bit offsets";
-            p = Plan();
             return HPMC_ERR_SIZE;
         }
-        cls_vol.insert(cls_vol.end(), wgs, (uint32_t)i);
-        tile_vol_l4.insert(tile_vol_l4.end(), d.n_l4tiles, (uint32_t)i);
-        tile_vol_col.insert(tile_vol_col.end(), d.n_coltiles, (uint32_t)i);
+        p.n_l4t += d.n_l4tiles;
+        p.n_colt += d.n_coltiles;
         p.n_cc += d.n_cc;
         p.n_l4 += d.n_level4;
         p.n_wg += (uint32_t)wgs;
         p.n_lattice += (uint64_t)d.g.sx * d.g.sy * d.g.sz;
         if (p.n_lattice >= 0xffffffffull) {
             *err = "build too large for 32-bit sample keys";
-            p = Plan();
             return HPMC_ERR_SIZE;
         }
         // a block holds at most 256 * 15 vertices: 15 work items per block slice
         p.n_items_max += (uint64_t)d.g.nbx * d.g.nby * d.g.cz * 15;
     }
-    p.n_l4t = (uint32_t)tile_vol_l4.size();
-    p.n_colt = (uint32_t)tile_vol_col.size();
-    std::vector<uint32_t> tile_vol = tile_vol_l4;
-    tile_vol.insert(tile_vol.end(), tile_vol_col.begin(), tile_vol_col.end());
 
     // persistent emit grid: as many 4-wave workgroups as fit at once, a
     // multiple of 8 for the XCD-aware item mapping, never more than the items
@@ -324,6 +317,37 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
     const uint64_t max_wg = (p.n_items_max + hpmc::kEmitWaves - 1) / hpmc::kEmitWaves;
     emit_wgs = (int)std::min<uint64_t>((uint64_t)emit_wgs, std::max<uint64_t>(max_wg, 1));
     p.emit_wgs = (emit_wgs + 7) / 8 * 8;
+    return HPMC_OK;
+}
+
+// Lays out and allocates a plan for `specs` (plan_describe, then the device
+// arrays).  Returns HPMC_OK or an error code with *err set (the plan is then
+// empty).
+inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const Tuning& tune,
+                           const void* classify_kernel, const DeviceShape& dev, bool stack_bands,
+                           std::string* err)
+{
+    plan_free(p);
+    p.layouts.resize(specs.size());
+    p.vols.resize(specs.size());
+    const int32_t rc = plan_describe(p, specs.data(), specs.size(), p.layouts.data(), p.vols.data(), tune,
+                                     classify_kernel, dev, stack_bands, err);
+    if (rc != HPMC_OK) {
+        p = Plan();
+        return rc;
+    }
+    // volume of each classification workgroup and scan tile (level-4 tiles
+    // of all volumes, then their column tiles)
+    std::vector<uint32_t> cls_vol, tile_vol;
+    cls_vol.reserve(p.n_wg);
+    tile_vol.reserve((size_t)p.n_l4t + p.n_colt);
+    for (size_t i = 0; i < specs.size(); ++i) {
+        const hpmc::VolDesc& d = p.vols[i];
+        const uint32_t next_wg = i + 1 < specs.size() ? p.vols[i + 1].cls_wg0 : p.n_wg;
+        cls_vol.insert(cls_vol.end(), next_wg - d.cls_wg0, (uint32_t)i);
+        tile_vol.insert(tile_vol.end(), d.n_l4tiles, (uint32_t)i);
+    }
+    for (size_t i = 0; i < specs.size(); ++i) tile_vol.insert(tile_vol.end(), p.vols[i].n_coltiles, (uint32_t)i);
 
     PLAN_TRY(hipMalloc(&p.d_vols, p.vols.size() * sizeof(hpmc::VolDesc)));
     PLAN_TRY(hipMemcpy(p.d_vols, p.vols.data(), p.vols.size() * sizeof(hpmc::VolDesc), hipMemcpyHostToDevice));
@@ -348,6 +372,28 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
     memset(p.h_vcounts, 0, p.vols.size() * sizeof(hpmc::VolCount));
     PLAN_TRY(hipDeviceSynchronize());
     return HPMC_OK;
+}
+
+// Device bytes of a one-volume plan's arrays carved from one buffer (each
+// array 256-byte aligned): descriptor, scan state, totals, volume count,
+// tile sums, column counts, block offsets, work items.
+inline uint64_t plan_carve(Plan& p, char* base)
+{
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        char* q = base ? base + off : nullptr;
+        off += (bytes + 255) / 256 * 256;
+        return q;
+    };
+    p.d_vols = (hpmc::VolDesc*)take(sizeof(hpmc::VolDesc));
+    p.scan_state = (hpmc::ScanState*)take(sizeof(hpmc::ScanState));
+    p.d_totals = (hpmc::Totals*)take(sizeof(hpmc::Totals));
+    p.d_vcounts = (hpmc::VolCount*)take(sizeof(hpmc::VolCount));
+    p.tile_sums = (uint2*)take((uint64_t)(p.n_l4t + p.n_colt) * sizeof(uint2));
+    p.colcnt = (uint16_t*)take(p.n_cc * sizeof(uint16_t));
+    p.l4_off = (uint32_t*)take(p.n_l4 * sizeof(uint32_t));
+    p.items = (uint2*)take(std::max<uint64_t>(p.n_items_max, 1) * sizeof(uint2));
+    return off;
 }
 
 // Replaces the field pointer of volume v (a new texture for the same

@@ -182,7 +182,9 @@ enum {
 int32_t HPMCgetLastError(struct HPMCHistoPyramid* h);
 const char* HPMCgetLastErrorString(struct HPMCHistoPyramid* h);
 
-/* HIP stream (hipStream_t) for all device work of h; NULL = null stream. */
+/* HIP stream (hipStream_t) for all device work of h; NULL = null stream.
+ * Work queued on the new stream afterwards runs after the work already
+ * queued on the old one (the new stream waits for an event of the old). */
 int32_t HPMCsetStream(struct HPMCHistoPyramid* h, void* hip_stream);
 
 /* Launch shapes of h's kernels, as HPMCbatchSetTuning; 0 picks
@@ -227,7 +229,10 @@ uint64_t HPMCgetLevel4(struct HPMCHistoPyramid* h, uint32_t* device_out, uint64_
 /* Per-stage timing with HIP events on the HistoPyramid's stream: when
  * enabled, every launch of classify, scan and emit is bracketed by a pair of
  * events.  HPMCgetStageTimes waits for them, writes the summed milliseconds
- * (ms[3] = classify, scan, emit) and launch counts, and starts over. */
+ * (ms[3] = classify, scan, emit) and launch counts, and starts over.  While
+ * profiling stays on without a HPMCgetStageTimes call, every 1024 pairs of a
+ * stage are folded into its totals (a wait on the device), so the events
+ * held stay bounded. */
 int32_t HPMCsetProfiling(struct HPMCHistoPyramid* h, int32_t enable);
 int32_t HPMCgetStageTimes(struct HPMCHistoPyramid* h, double* ms, uint32_t* launches);
 
