@@ -92,7 +92,11 @@ SIGNATURES = [
     ("HPMCbatchCaptureGraph", _i32, [_vp, _f, _vp, _u64, _vp]),
     ("HPMCbatchLaunchGraph", _i32, [_vp]),
     ("HPMCbatchAutotune", _i32, [_vp, _f, _vp, _u64, _i32, ctypes.POINTER(ctypes.c_double)]),
+    ("HPMCextractVolumeNoAlloc", _i32, [_vp, _vp, _i32, _i32, _i32, _f, _f, _f, _f, _vp, _u64, _vp, _vp]),
 ]
+
+# device scratch of HPMCextractVolumeNoAlloc (HPMC_NO_ALLOC_SCRATCH_BYTES)
+NO_ALLOC_SCRATCH_BYTES = 80 << 20
 
 _LIB = None
 
@@ -199,6 +203,36 @@ def _vertex_grads(grad_vertices, count, device):
 
 def _stream_ptr(stream):
     return ctypes.c_void_p(stream.cuda_stream) if stream is not None else None
+
+
+def constants():
+    """The process's HPMCConstants handle (created on first use, with the
+    current device current)."""
+    if IsoSurface._constants is None:
+        c = lib().HPMCcreateConstants(4, 3)
+        if not c:
+            raise HPMCError("HPMCcreateConstants failed: %s" % (last_error()[1],))
+        IsoSurface._constants = c
+    return IsoSurface._constants
+
+
+def extract_volume_no_alloc(volume, iso, out, count=None, extent=(1.0, 1.0, 1.0), stream=None):
+    """Build + extraction of a float32 device volume (z, y, x) in one call
+    that allocates nothing (HPMCextractVolumeNoAlloc): N3F_V3F rows go to
+    `out` (float32 (capacity, 6) device tensor, extra vertices dropped), the
+    vertex count to `count` (a one-element int32 device tensor) if given.
+    Enqueued on `stream` (torch stream or None); no host wait.  Calls that
+    overlap on one device race for its scratch."""
+    import torch
+    assert volume.dtype == torch.float32 and volume.is_contiguous() and volume.is_cuda
+    assert out.dtype == torch.float32 and out.is_contiguous() and out.is_cuda and out.dim() == 2 and out.shape[1] == 6
+    assert count is None or (count.dtype == torch.int32 and count.is_cuda and count.numel() >= 1)
+    z, y, x = volume.shape
+    code = lib().HPMCextractVolumeNoAlloc(constants(), ctypes.c_void_p(volume.data_ptr()), x, y, z,
+                                          *[float(e) for e in extent], float(iso), ctypes.c_void_p(out.data_ptr()),
+                                          out.shape[0], ctypes.c_void_p(count.data_ptr()) if count is not None else None,
+                                          _stream_ptr(stream))
+    _check(code)
 
 
 class IsoSurface:

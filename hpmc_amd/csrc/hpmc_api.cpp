@@ -89,6 +89,20 @@ extern "C" __global__ __launch_bounds__(256) void hpmc_fill_f32(float* p, uint64
         p[i] = v;
 }
 
+// Start of a no-allocation build: the volume's descriptor into its scratch
+// slot (passed by value, so a captured graph carries it), the scan tile sums
+// and the scan's counter cleared.
+extern "C" __global__ __launch_bounds__(256) void hpmc_noalloc_begin(hpmc::VolDesc d, hpmc::VolDesc* dst,
+                                                                     uint2* tile_sums, uint32_t ntiles,
+                                                                     hpmc::ScanState* state)
+{
+    if (threadIdx.x == 0) {
+        *dst = d;
+        state->done = 0u;
+    }
+    for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) tile_sums[i] = make_uint2(0u, 0u);
+}
+
 #include "hpmc_plan.h"
 
 namespace hpmc_host {
@@ -546,6 +560,44 @@ const hpmc_host::DeviceShape& device_shape()
     return shape;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point (HPMCextractVolumeNoAlloc): the
+// device arrays of a one-volume plan carved out of one file-scope device
+// array.  Each device holds its own copy (allocated by the runtime when it
+// loads this library's code on that device, kept until the process ends);
+// in host code the name is only a handle, the device address comes from
+// hipGetSymbolAddress, once per device.  Every call on a device shares that
+// device's copy: calls that overlap on one device race.
+constexpr uint64_t kNoAllocScratchBytes = 80ull << 20;
+static_assert(kNoAllocScratchBytes == HPMC_NO_ALLOC_SCRATCH_BYTES, "header and library agree");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+struct ScratchAddress {
+    std::once_flag once;
+    void* ptr = nullptr;
+    hipError_t status = hipErrorNotInitialized;
+};
+ScratchAddress g_scratch_address[kMaxDevices];
+
+// The address of `scratch` on device `dev`, which must be the current
+// device; looked up on the first call for that device, null on failure
+// (*status says why).
+void* scratch_address(int dev, hipError_t* status)
+{
+    if (dev < 0 || dev >= kMaxDevices) {
+        *status = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    ScratchAddress& a = g_scratch_address[dev];
+    std::call_once(a.once, [&a] {
+        a.status = hipGetSymbolAddress(&a.ptr, HIP_SYMBOL(scratch));
+        if (a.status != hipSuccess) a.ptr = nullptr;
+    });
+    *status = a.status;
+    return a.ptr;
+}
+
 // Uploads the case and binary-normal tables once; false without a device.
 bool upload_tables(HPMCConstants* c)
 {
@@ -568,6 +620,10 @@ bool upload_tables(HPMCConstants* c)
     c->device = dev;
     c->d_cases = cases;
     c->d_bnorm = bn;
+    // this device's copy of the no-allocation entry point's scratch array
+    // is looked up now, at set-up, with the device current
+    hipError_t st;
+    (void)scratch_address(dev, &st);
     return true;
 }
 
@@ -1628,6 +1684,58 @@ int32_t HPMCbatchAutotune(struct HPMCBatch* b, float threshold, float* out, uint
     rc = batch_setup(b);
     if (rc != HPMC_OK) return rc;
     if (best_ms) *best_ms = best_t;
+    ok(nullptr);
+    return HPMC_OK;
+}
+
+// No-allocation build + extraction of an FP32 volume (no counterpart in
+// the reference): the kernels of HPMCbuildHistopyramid + HPMCextractVertices
+// with the plan's arrays in `scratch`.
+int32_t HPMCextractVolumeNoAlloc(struct HPMCConstants* c, const float* volume, int32_t x, int32_t y, int32_t z,
+                                 float ext_x, float ext_y, float ext_z, float threshold, float* out,
+                                 uint64_t capacity, uint32_t* device_count, void* hip_stream)
+{
+    if (!c || !volume || (!out && capacity)) return fail(nullptr, HPMC_ERR_NULL, "NULL argument");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(nullptr, HPMC_ERR_HIP, "no HIP device");
+    if (!c->d_cases || c->device != dev)
+        return fail(nullptr, HPMC_ERR_ARG, "the constants were not created with device %d current", dev);
+    hipError_t st = hipSuccess;
+    char* base = (char*)scratch_address(dev, &st);
+    if (!base) return fail(nullptr, HPMC_ERR_HIP, "scratch address on device %d: %s", dev, hipGetErrorString(st));
+    hpmc_host::VolumeSpec spec{{x, y, z}, {x - 1, y - 1, z - 1}, {ext_x, ext_y, ext_z}, volume};
+    hpmc_host::Plan p;   // default-constructed: no host or device memory
+    hpmc_host::Layout lay;
+    hpmc::VolDesc desc;
+    std::string err;
+    int32_t rc = hpmc_host::plan_describe(p, &spec, 1, &lay, &desc, hpmc_host::Tuning{},
+                                          (const void*)hpmc_classify_volume, device_shape(), true, &err);
+    if (rc != HPMC_OK) return fail(nullptr, rc, "%s", err.c_str());
+    const uint64_t need = hpmc_host::plan_carve(p, nullptr);
+    if (need > kNoAllocScratchBytes)
+        return fail(nullptr, HPMC_ERR_SIZE,
+                    "lattice %dx%dx%d needs %llu bytes of scratch; the no-allocation entry point holds %llu",
+                    x, y, z, (unsigned long long)need, (unsigned long long)kNoAllocScratchBytes);
+    (void)hpmc_host::plan_carve(p, base);
+    const hipStream_t stream = (hipStream_t)hip_stream;
+    hipLaunchKernelGGL(hpmc_noalloc_begin, dim3(1), dim3(256), 0, stream, desc, p.d_vols, p.tile_sums,
+                       p.n_l4t + p.n_colt, p.scan_state);
+    // lattices past the MALL stream through classification (as setup picks)
+    const uint64_t bytes = 4ull * (uint64_t)x * (uint64_t)y * (uint64_t)z;
+    const hpmc_host::Kernel k_cls{bytes >= (256ull << 20) ? (const void*)hpmc_classify_volume_stream
+                                                           : (const void*)hpmc_classify_volume,
+                                  nullptr};
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(capacity, 0xffffffffull);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hpmc_host::plan_classify(p, k_cls, threshold, c->d_cases, stream);
+    if (e == hipSuccess) e = hpmc_host::plan_scan(p, (const void*)hpmc_scan, stream);
+    if (e == hipSuccess)
+        e = hpmc_host::plan_emit(p, hpmc_host::Kernel{(const void*)hpmc_emit_volume, nullptr}, threshold, out,
+                                 nullptr, cap, c->d_cases, c->d_bnorm, stream);
+    if (e == hipSuccess && device_count)
+        e = hipMemcpyAsync(device_count, &p.d_totals->vertices, sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return fail(nullptr, HPMC_ERR_HIP, "no-allocation build: %s", hipGetErrorString(e));
     ok(nullptr);
     return HPMC_OK;
 }

@@ -321,6 +321,31 @@ int32_t HPMCbatchAdjoint(struct HPMCBatch* b, const float* grad_vertices, float*
 int32_t HPMCbatchAutotune(struct HPMCBatch* b, float threshold, float* out, uint64_t capacity, int32_t repetitions,
                           double* best_ms);
 
+/* ---- build + extraction without allocation ------------------------------
+ *
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  One call does what HPMCbuildHistopyramid (continuous
+ * field) + HPMCextractVertices do for an FP32 volume (device lattice x*y*z,
+ * x fastest, grid = lattice - 1, extent as HPMCsetGridExtent): the same
+ * kernels, the same vertices in the same order with the same bits.  Writes
+ * N3F_V3F rows into `out` (capacity vertices; the rest are dropped) and, if
+ * device_count is not NULL, the vertex count into that device word.  All of
+ * it is enqueued on hip_stream (NULL = null stream); nothing waits on the
+ * host.  It calls no hipMalloc, hipFree or other allocating routine: the
+ * build's arrays live in a scratch array of HPMC_NO_ALLOC_SCRATCH_BYTES per
+ * device, held from the first use of the library on that device until the
+ * process ends; a lattice whose arrays need more is rejected with
+ * HPMC_ERR_SIZE before anything is enqueued (512^3 needs about 66 MB).
+ * Every call on a device shares that device's scratch, so calls that overlap
+ * on one device (other streams, other threads) race: order them.  `c` must
+ * have been created with the current device current.  Errors as for
+ * batches: HPMCgetLastError(NULL) of the calling thread. */
+#define HPMC_NO_ALLOC_SCRATCH_BYTES (80ull << 20)
+int32_t HPMCextractVolumeNoAlloc(struct HPMCConstants* c, const float* volume, int32_t x, int32_t y, int32_t z,
+                                 float ext_x, float ext_y, float ext_z, float threshold, float* out,
+                                 uint64_t capacity, uint32_t* device_count, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif
