@@ -1001,7 +1001,14 @@ struct HPMCTraversalHandle* HPMCcreateTraversalHandle(struct HPMCHistoPyramid* h
         fail(nullptr, HPMC_ERR_NULL, "createTraversalHandle called with h == NULL");
         return nullptr;
     }
-    if (!setup(h)) return nullptr;
+    if (!setup(h)) {
+        // Without a HIP device the handle is still made, as the constants
+        // are (its traversal source can be taken on the host; extraction
+        // fails).  Any other setup failure is one, as in the reference.
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) return nullptr;
+        (void)hipGetLastError();
+    }
     HPMCTraversalHandle* th = new (std::nothrow) HPMCTraversalHandle;
     if (!th) return nullptr;
     th->h = h;
@@ -1063,7 +1070,11 @@ char* HPMCgetTraversalShaderFunctions(struct HPMCTraversalHandle* th)
          "\n";
     s += "// cb(k, normal, position) for every vertex k of the build the view\n"
          "// describes (HPMCgetTraversalView); CB needs a static constexpr bool\n"
-         "// kActive = true.  Launch HPMCgetTraversalGrid workgroups of 256 threads.\n"
+         "// kActive = true.  With static constexpr bool kWantsEdge = true as well,\n"
+         "// cb(k, a, b, normal, position) gets the texel-centre coordinates a, b\n"
+         "// ((i + 0.5) / lattice size) of the vertex's edge ends too, the\n"
+         "// reference's extractVertex(a, b, p, n).  Launch HPMCgetTraversalGrid\n"
+         "// workgroups of 256 threads.\n"
          "template <class CB>\n"
          "__device__ __forceinline__ void hpmc_for_each_vertex(const hpmc::TraversalView& v, const CB& cb)\n"
          "{\n"

@@ -913,9 +913,24 @@ struct AdjointIO {
 // Per-vertex callback of user-fused traversal kernels: emission calls
 // cb(k, normal, position) for output vertex k instead of writing the
 // output buffer.  The default does nothing and leaves emission as it is.
+// A callback that declares kWantsEdge = true is called as
+// cb(k, a, b, normal, position) instead, with a and b the texel-centre
+// coordinates ((i + 0.5) / lattice size per axis, the coordinates a field
+// snippet's HPMC_fetch takes) of the two ends of the vertex's edge: the
+// reference's four-output extractVertex(a, b, p, n)
+// (hpmc/src/shadergen.cpp:310, 390-391).
 struct NoVertexCallback {
     static constexpr bool kActive = false;
     __device__ void operator()(uint32_t, float3, float3) const {}
+};
+
+template <class CB, class = void>
+struct wants_edge {
+    static constexpr bool value = false;
+};
+template <class CB>
+struct wants_edge<CB, decltype((void)CB::kWantsEdge, void())> {
+    static constexpr bool value = CB::kWantsEdge;
 };
 
 // Lattice planes an emit item stages: z, z+1 for the corners, z+2 for the
@@ -1358,10 +1373,23 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
             // a user-fused traversal: hand each vertex over, write nothing
             for (uint32_t v0 = vlo; v0 < vhi; v0 += 64) {
                 const uint32_t v = min(v0 + (uint32_t)lane, vhi - 1u);
+                const Site st = site_of(v);
                 float o[6];
-                eval(site_of(v), o);
-                if (v0 + (uint32_t)lane < vhi)
-                    cb(base + v, make_float3(o[0], o[1], o[2]), make_float3(o[3], o[4], o[5]));
+                eval(st, o);
+                if (v0 + (uint32_t)lane < vhi) {
+                    if constexpr (wants_edge<CB>::value) {
+                        // edge ends as texel-centre coordinates (i + 0.5) / size
+                        const int ax = cx0 + st.ax, ay = cy0 + st.ay, az = cz + st.az;
+                        const int bx = ax + (st.axis == 0), by = ay + (st.axis == 1), bz = az + (st.axis == 2);
+                        const float3 a = make_float3(((float)ax + 0.5f) * g.ipx, ((float)ay + 0.5f) * g.ipy,
+                                                     ((float)az + 0.5f) * g.ipz);
+                        const float3 b = make_float3(((float)bx + 0.5f) * g.ipx, ((float)by + 0.5f) * g.ipy,
+                                                     ((float)bz + 0.5f) * g.ipz);
+                        cb(base + v, a, b, make_float3(o[0], o[1], o[2]), make_float3(o[3], o[4], o[5]));
+                    } else {
+                        cb(base + v, make_float3(o[0], o[1], o[2]), make_float3(o[3], o[4], o[5]));
+                    }
+                }
             }
         } else {
             for (uint32_t v0 = vlo; v0 < vhi; v0 += 64) {

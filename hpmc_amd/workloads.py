@@ -120,6 +120,119 @@ def cayley(size, extra=0):
 
 
 # ---------------------------------------------------------------------------
+# The particles demo's algebraic shapes (apps/particles/particles.cpp:100-158):
+# a degree-5 polynomial in q = 2.2 (p - 1/2) with twelve coefficients in the
+# uniform `shape` (set with HPMCsetFieldUniform, as the demo sets it with
+# glUniform1fv at particles.cpp:728), negated.  HPMC_fetchGrad keeps the
+# demo's expressions as written, including its z-derivative of the y^2 z^2
+# term, 2 shape[6] y z^2 (particles.cpp:142), and gives the gradient in q.
+PARTICLES_SNIPPET = r"""
+__device__ float shape[12];
+__device__ float HPMC_fetch(float3 p)
+{
+    const float x = (p.x - 0.5f) * 2.2f;
+    const float y = (p.y - 0.5f) * 2.2f;
+    const float z = (p.z - 0.5f) * 2.2f;
+    return -(shape[0] * x * x * x * x * x + shape[1] * x * x * x * x + shape[2] * y * y * y * y +
+             shape[3] * z * z * z * z + shape[4] * x * x * y * y + shape[5] * x * x * z * z +
+             shape[6] * y * y * z * z + shape[7] * x * y * z + shape[8] * x * x + shape[9] * y * y +
+             shape[10] * z * z + shape[11]);
+}
+"""
+
+PARTICLES_GRAD_SNIPPET = PARTICLES_SNIPPET + r"""
+__device__ float4 HPMC_fetchGrad(float3 p)
+{
+    const float x = (p.x - 0.5f) * 2.2f;
+    const float y = (p.y - 0.5f) * 2.2f;
+    const float z = (p.z - 0.5f) * 2.2f;
+    const float gx = 5.0f * shape[0] * x * x * x * x + 4.0f * shape[1] * x * x * x + 2.0f * shape[4] * x * y * y +
+                     2.0f * shape[5] * x * z * z + shape[7] * y * z + 2.0f * shape[8] * x;
+    const float gy = 4.0f * shape[2] * y * y * y + 2.0f * shape[4] * x * x * y + 2.0f * shape[6] * y * z * z +
+                     shape[7] * x * z + 2.0f * shape[9] * y;
+    const float gz = 4.0f * shape[3] * z * z * z + 2.0f * shape[5] * x * x * z + 2.0f * shape[6] * y * z * z +
+                     shape[7] * x * y + 2.0f * shape[10] * z;
+    return make_float4(-gx, -gy, -gz, HPMC_fetch(p));
+}
+"""
+
+# Coefficients of the demo's shapes (particles.cpp:697-712: x^5, x^4, y^4,
+# z^4, x^2y^2, x^2z^2, y^2z^2, xyz, x^2, y^2, z^2, 1) and its iso-value.
+PARTICLES_SHAPES = {
+    "helix": (0.0, -2.0, 0.0, 0.0, 0.0, 0.0, -1.0, 0.0, 6.0, 0.0, 0.0, 0.0),
+    "between1": (0.0, 8.0, 0.5, 0.5, 4.0, 4.0, -1.4, 0.0, 0.0, 0.0, 0.0, 0.0),
+    "between2": (0.0, 16.0, 1.0, 1.0, 8.0, 8.0, -2.0, 0.0, -6.0, 0.0, 0.0, 0.0),
+    "daddel": (0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 1.0, 0.3, -0.95),
+    "torus": (0.0, 1.0, 1.0, 1.0, 2.0, 2.0, 2.0, 0.0, -1.01125, -1.01125, 0.94875, 0.225032),
+    "kiss": (-0.5, -0.5, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 1.0, 0.0),
+    "cayley": (0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 16.0, 4.0, 4.0, 4.0, -1.0),
+}
+PARTICLES_ISO = 0.001
+
+
+def particles_shape(a, b, u):
+    """The demo's morph between two shapes, (1 - u) C[a] + u C[b] in FP32
+    (particles.cpp:714-720)."""
+    f32 = np.float32
+    ca = np.asarray(PARTICLES_SHAPES[a], np.float32)
+    cb = np.asarray(PARTICLES_SHAPES[b], np.float32)
+    return ((f32(1.0) - f32(u)) * ca + f32(u) * cb).astype(np.float32)
+
+
+def _particles_q(size, extra):
+    px, py, pz = _grid(size, extra)
+    f32 = np.float32
+    return (px - f32(0.5)) * f32(2.2), (py - f32(0.5)) * f32(2.2), (pz - f32(0.5)) * f32(2.2)
+
+
+def particles(size, shape, extra=0):
+    """PARTICLES_SNIPPET's HPMC_fetch at every lattice point (z, y, x), in
+    the snippet's FP32 operation order (products and sums left to right)."""
+    s = [np.float32(v) for v in np.asarray(shape, np.float32)]
+    x, y, z = _particles_q(size, extra)
+    t = s[0] * x * x * x * x * x + s[1] * x * x * x * x
+    t = t + s[2] * y * y * y * y
+    t = t + s[3] * z * z * z * z
+    t = t + s[4] * x * x * y * y
+    t = t + s[5] * x * x * z * z
+    t = t + s[6] * y * y * z * z
+    t = t + s[7] * x * y * z
+    t = t + s[8] * x * x
+    t = t + s[9] * y * y
+    t = t + s[10] * z * z
+    t = t + s[11]
+    return (-t).astype(np.float32)
+
+
+def particles_grad(size, shape, extra=0):
+    """(Z, Y, X, 4) float32: PARTICLES_GRAD_SNIPPET's HPMC_fetchGrad in its
+    FP32 operation order."""
+    s = [np.float32(v) for v in np.asarray(shape, np.float32)]
+    f32 = np.float32
+    x, y, z = _particles_q(size, extra)
+    gx = f32(5.0) * s[0] * x * x * x * x + f32(4.0) * s[1] * x * x * x
+    gx = gx + f32(2.0) * s[4] * x * y * y
+    gx = gx + f32(2.0) * s[5] * x * z * z
+    gx = gx + s[7] * y * z
+    gx = gx + f32(2.0) * s[8] * x
+    gy = f32(4.0) * s[2] * y * y * y + f32(2.0) * s[4] * x * x * y
+    gy = gy + f32(2.0) * s[6] * y * z * z
+    gy = gy + s[7] * x * z
+    gy = gy + f32(2.0) * s[9] * y
+    gz = f32(4.0) * s[3] * z * z * z + f32(2.0) * s[5] * x * x * z
+    gz = gz + f32(2.0) * s[6] * y * z * z
+    gz = gz + s[7] * x * y
+    gz = gz + f32(2.0) * s[10] * z
+    f = particles(size, shape, extra)
+    out = np.zeros(f.shape + (4,), np.float32)
+    out[..., 0] = -np.broadcast_to(gx, f.shape)
+    out[..., 1] = -np.broadcast_to(gy, f.shape)
+    out[..., 2] = -np.broadcast_to(gz, f.shape)
+    out[..., 3] = f
+    return out
+
+
+# ---------------------------------------------------------------------------
 # Metaballs: f(p) = sum_i r_i^2 / |p - c_i|^2, iso 1.
 METABALL_SNIPPET = r"""
 __device__ int hpmc_nballs;

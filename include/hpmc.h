@@ -111,7 +111,10 @@ uint32_t HPMCacquireNumberOfVertices(struct HPMCHistoPyramid* h);
 
 /* ---- traversal and extraction ------------------------------------------ */
 
-/* Replaces HPMCcreateTraversalHandle (hpmc.h:477-478, triface.cpp:43-109). */
+/* Replaces HPMCcreateTraversalHandle (hpmc.h:477-478, triface.cpp:43-109).
+ * Like HPMCcreateConstants, it still makes the handle when no HIP device is
+ * present (so its traversal source can be taken on the host); extraction
+ * then fails. */
 struct HPMCTraversalHandle* HPMCcreateTraversalHandle(struct HPMCHistoPyramid* h);
 
 /* Replaces HPMCdestroyTraversalHandle (hpmc.h:484-485, triface.cpp:112-122). */
@@ -120,7 +123,11 @@ void HPMCdestroyTraversalHandle(struct HPMCTraversalHandle* th);
 /* Replaces HPMCgetTraversalShaderFunctions (hpmc.h:493-494,
  * triface.cpp:125-192): returns HIP device source (layout constants plus
  * the device functions) for callers that fuse extraction into their own
- * hipRTC kernels.  The caller frees it with free(). */
+ * hipRTC kernels.  The caller frees it with free().  Its
+ * hpmc_for_each_vertex(view, cb) calls cb(k, normal, position), or, for a
+ * callback declaring kWantsEdge = true, cb(k, a, b, normal, position) with
+ * the texel-centre coordinates of the vertex's edge ends: the reference's
+ * two- and four-output extractVertex forms (shadergen.cpp:310, 443). */
 char* HPMCgetTraversalShaderFunctions(struct HPMCTraversalHandle* th);
 
 /* Bytes of the traversal view (hpmc::TraversalView in the source above). */

@@ -92,10 +92,41 @@ def test_bad_snippet_reports_compile_error():
         hpmc_amd.check_snippet("__device__ float not_the_fetch(float3 p) { return 0.0f; }")
 
 
-def test_traversal_source_is_valid_hip():
-    # the device source HPMCgetTraversalShaderFunctions hands out compiles
-    src = open(os.path.join(ROOT, "hpmc_amd", "csrc", "hpmc_device.h")).read()
-    assert "emit_body" in src and "classify_body" in src
+@pytest.mark.parametrize("kind", ["volume", "gradient_volume", "unorm8", "binary", "particles",
+                                  "particles_gradient"])
+def test_traversal_source_compiles_with_user_kernels(kind):
+    # HPMCgetTraversalShaderFunctions' own output, for each field kind, with a
+    # two-output and a four-output (edge ends) user kernel, compiled for
+    # gfx950 by hipRTC -- no device involved (the volume pointer is never read)
+    from rtc_util import USER_EDGE_KERNEL, USER_KERNEL, compile_hip
+    L = hpmc_amd.lib()
+    iso = hpmc_amd.IsoSurface((40, 36, 30))
+    dummy = ctypes.c_void_p(256)
+    if kind == "volume":
+        L.HPMCsetFieldTexture3D(iso.h, dummy, 0)
+    elif kind == "gradient_volume":
+        L.HPMCsetFieldTexture3D(iso.h, dummy, 1)
+    elif kind == "unorm8":
+        L.HPMCsetFieldTexture3DUnorm8(iso.h, dummy)
+    elif kind == "binary":
+        iso.set_binary(True)
+        L.HPMCsetFieldTexture3D(iso.h, dummy, 0)
+    elif kind == "particles":
+        iso.set_snippet(workloads.PARTICLES_SNIPPET)
+    else:
+        iso.set_snippet(workloads.PARTICLES_GRAD_SNIPPET, gradient=True)
+    src = iso.traversal_source()
+    assert "hpmc_for_each_vertex" in src and "#define HPMC_TILES_X" in src
+    code, log = compile_hip(src + USER_KERNEL + USER_EDGE_KERNEL)
+    assert code is not None and len(code) > 1000, log[-3000:]
+    iso.close()
+
+
+@pytest.mark.parametrize("gradient", [False, True])
+def test_particles_snippets_compile_for_gfx950(gradient):
+    src = workloads.PARTICLES_GRAD_SNIPPET if gradient else workloads.PARTICLES_SNIPPET
+    hpmc_amd.check_snippet(src, gradient=gradient)
+    hpmc_amd.check_snippet(src, gradient=gradient, binary=True)
 
 
 def test_output_buffer_growth_policy():
@@ -158,7 +189,8 @@ def test_layout_rejects_bad_sizes(lattice, grid):
 
 def test_build_without_device_fails_loudly():
     # no CPU fallback: the build reports a HIP error and latches broken
-    # (hpiface.cpp:258-290), the count is 0, no traversal handle is made.
+    # (hpiface.cpp:258-290), the count is 0; a traversal handle is still
+    # made (its source can be taken on the host) but extraction fails.
     # (Asked inside the test, through torch: nothing may start a HIP runtime
     # while the tests are collected.)
     torch = pytest.importorskip("torch")
@@ -169,7 +201,10 @@ def test_build_without_device_fails_loudly():
     L.HPMCbuildHistopyramid(h, 0.0)
     assert L.HPMCgetLastError(h) == 6           # HPMC_ERR_HIP
     assert L.HPMCacquireNumberOfVertices(h) == 0
-    assert not L.HPMCcreateTraversalHandle(h)
+    th = L.HPMCcreateTraversalHandle(h)
+    assert th
+    assert not L.HPMCextractVertices(th)
+    L.HPMCdestroyTraversalHandle(th)
     L.HPMCdestroyHandle(h)
     L.HPMCdestroyConstants(c)
 

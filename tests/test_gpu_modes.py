@@ -135,3 +135,30 @@ def test_kept_output_buffer_grows_like_the_reference_demo():
         ptr = kept.data_ptr()
     iso.close()
     ref.close()
+
+
+# The particles demo's degree-5 shapes (apps/particles/particles.cpp:100-158)
+# as hipRTC snippets with their 12-float `shape` uniform: continuous mode
+# (forward-difference normals) and the demo's own gradient mode
+# (HPMC_fetchGrad, with its z-derivative as written), iso 0.001.
+@pytest.mark.parametrize("lattice,shape", [((128, 128, 128), ("torus", "torus", 0.0)),
+                                           ((96, 112, 80), ("helix", "between1", 0.4)),
+                                           ((100, 90, 110), ("between2", "daddel", 0.5))])
+def test_particles_snippet(lattice, shape):
+    coeffs = workloads.particles_shape(*shape)
+    T = workloads.PARTICLES_ISO
+    res = gpu_run(lattice, T, snippet=workloads.PARTICLES_SNIPPET, uniforms=[("shape", coeffs)])
+    f = workloads.particles(lattice, coeffs, extra=2)
+    assert check(res, f, T, lattice, clamp=False) > 10000
+
+
+@pytest.mark.parametrize("lattice,shape", [((128, 128, 128), ("helix", "between1", 0.4)),
+                                           ((90, 100, 70), ("daddel", "daddel", 0.0))])
+def test_particles_gradient_snippet(lattice, shape):
+    coeffs = workloads.particles_shape(*shape)
+    T = workloads.PARTICLES_ISO
+    res = gpu_run(lattice, T, snippet=workloads.PARTICLES_GRAD_SNIPPET, uniforms=[("shape", coeffs)],
+                  gradient=True)
+    f4 = workloads.particles_grad(lattice, coeffs, extra=2)
+    f = np.ascontiguousarray(f4[..., 3])
+    assert check(res, f, T, lattice, clamp=False, mode="gradient", field4=f4) > 10000

@@ -13,58 +13,25 @@ import pytest
 
 from hpmc_amd import IsoSurface, workloads
 from parity_util import smooth_noise, with_gradient
+from rtc_util import USER_EDGE_KERNEL, USER_KERNEL, compile_hip
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
-USER_KERNEL = r"""
-struct StoreVertex {
-    static constexpr bool kActive = true;
-    float* out;
-    unsigned* seen;
-    __device__ void operator()(uint32_t k, float3 n, float3 p) const {
-        float* o = out + 6 * (size_t)k;
-        o[0] = n.x; o[1] = n.y; o[2] = n.z; o[3] = p.x; o[4] = p.y; o[5] = p.z;
-        atomicAdd(seen + k, 1u);
-    }
-};
-extern "C" __global__ __launch_bounds__(256) void user_store(hpmc::TraversalView v, float* out, unsigned* seen)
-{
-    hpmc_for_each_vertex(v, StoreVertex{out, seen});
-}
-"""
-
-# the options the library compiles its own snippets with
-RTC_OPTS = [b"--offload-arch=gfx950", b"-O3", b"-std=c++17", b"-ffp-contract=off",
-            b"-Xclang", b"-target-feature", b"-Xclang", b"-packed-fp32-ops"]
-
 
 class _Hip:
-    """hipRTC and the HIP module API through ctypes (the runtime the
+    """The HIP module API through ctypes (the runtime the
     library itself is linked against)."""
 
     def __init__(self):
         import hpmc_amd
         hpmc_amd.lib()
-        self.rtc = ctypes.CDLL("libhiprtc.so.7")
         self.hip = ctypes.CDLL("libamdhip64.so.7")
         self.modules = []
 
     def compile(self, source, name):
-        rtc = self.rtc
-        prog = ctypes.c_void_p()
-        assert rtc.hiprtcCreateProgram(ctypes.byref(prog), source.encode(), b"user.hip", 0, None, None) == 0
-        opts = (ctypes.c_char_p * len(RTC_OPTS))(*RTC_OPTS)
-        r = rtc.hiprtcCompileProgram(prog, len(RTC_OPTS), opts)
-        n = ctypes.c_size_t()
-        rtc.hiprtcGetProgramLogSize(prog, ctypes.byref(n))
-        log = ctypes.create_string_buffer(n.value + 1)
-        rtc.hiprtcGetProgramLog(prog, log)
-        assert r == 0, log.value.decode(errors="replace")[-4000:]
-        rtc.hiprtcGetCodeSize(prog, ctypes.byref(n))
-        code = ctypes.create_string_buffer(n.value)
-        assert rtc.hiprtcGetCode(prog, code) == 0
-        rtc.hiprtcDestroyProgram(ctypes.byref(prog))
+        code, log = compile_hip(source)
+        assert code is not None, log[-4000:]
         module = ctypes.c_void_p()
         assert self.hip.hipModuleLoadData(ctypes.byref(module), code) == 0
         self.modules.append(module)
@@ -155,4 +122,35 @@ def test_any_grid_and_rebuilds(hip):
     assert ref2.shape[0] != ref.shape[0]
     got2, seen2 = _visit(hip, iso)
     assert bool((seen2 == 1).all()) and torch.equal(got2, ref2)
+    iso.close()
+
+
+@pytest.mark.parametrize("kind", ["volume", "snippet"])
+def test_four_output_form_hands_the_edge_ends(hip, kind):
+    # the reference's extractVertex(a, b, p, n) (shadergen.cpp:310,
+    # 390-391): a and b are the texel-centre coordinates (i + 0.5) / size of
+    # the vertex's edge ends, the coordinates HPMC_fetch takes
+    iso, keep, T = _case(kind)
+    ref, ce = iso.extract_all(T, with_cell_edge=True)
+    n = ref.shape[0]
+    assert n > 500
+    fn = hip.compile(iso.traversal_source() + USER_EDGE_KERNEL, b"user_store_edge")
+    out = torch.full((n, 6), float("nan"), device="cuda")
+    ends = torch.full((n, 6), float("nan"), device="cuda")
+    seen = torch.zeros(n, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    hip.launch(fn, iso.traversal_grid(), iso.traversal_view(), out.data_ptr(), ends.data_ptr(), seen.data_ptr())
+    assert bool((seen == 1).all()) and torch.equal(out, ref)
+    ce = ce.cpu().numpy().view(np.uint32)
+    cells = tuple(v - 1 for v in iso.lattice)
+    cell, edge = ce[:, 0].astype(np.int64), ce[:, 1].astype(np.int64)
+    x, y, z = cell % cells[0], (cell // cells[0]) % cells[1], cell // (cells[0] * cells[1])
+    axis, s0, s1 = edge >> 2, edge & 1, (edge >> 1) & 1
+    a = np.stack([x + np.where(axis == 0, 0, s0), y + np.where(axis == 0, s0, np.where(axis == 1, 0, s1)),
+                  z + np.where(axis == 2, 0, s1)], axis=1)
+    b = a + np.eye(3, dtype=np.int64)[axis]
+    inv = np.array([np.float32(1.0) / np.float32(v) for v in iso.lattice], np.float32)
+    want = np.concatenate([(a.astype(np.float32) + np.float32(0.5)) * inv,
+                           (b.astype(np.float32) + np.float32(0.5)) * inv], axis=1)
+    assert np.array_equal(ends.cpu().numpy(), want)
     iso.close()
