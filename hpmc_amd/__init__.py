@@ -415,10 +415,13 @@ class IsoSurface:
         return codes.view(self.grid[2], self.grid[1], self.grid[0])
 
     def level4(self):
+        """Block sums of the last build by Morton block code (16x16-cell
+        blocks, narrower ones for grids of at most 8 cells along x or y)."""
         import torch
-        lay = self.layout()
-        side = (2 << lay[4]) // 16
-        buf = torch.empty(side * side, dtype=torch.int32, device="cuda")
+        n = lib().HPMCgetLevel4(self.h, None, 0)
+        if n == 0:
+            raise HPMCError("level4 failed: %s" % (last_error(self.h)[1],))
+        buf = torch.empty(n, dtype=torch.int32, device="cuda")
         n = lib().HPMCgetLevel4(self.h, ctypes.c_void_p(buf.data_ptr()), buf.numel())
         if n == 0:
             raise HPMCError("level4 failed: %s" % (last_error(self.h)[1],))

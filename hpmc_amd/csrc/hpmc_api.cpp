@@ -320,6 +320,7 @@ struct HPMCHistoPyramid {
     // device state of the build (one volume)
     hpmc_host::Plan plan;
     hpmc_host::Kernel k_classify, k_emit, k_codes;
+    hpmc_host::Kernel k_classify_small;   // grids with blocks narrower than 16 cells
     hipEvent_t ev_totals = nullptr;  // after the scan: totals are in pinned memory
     hpmc_host::AdjointScratch adjoint;
     bool built = false;
@@ -398,7 +399,7 @@ void free_device(HPMCHistoPyramid* h)
     if (h->module) (void)hipModuleUnload(h->module);
     h->ev_totals = nullptr;
     h->module = nullptr;
-    h->k_classify = h->k_emit = h->k_codes = hpmc_host::Kernel();
+    h->k_classify = h->k_emit = h->k_codes = h->k_classify_small = hpmc_host::Kernel();
     h->built = false;
     h->count_valid = false;
 }
@@ -542,6 +543,7 @@ bool compile_snippet(HPMCHistoPyramid* h)
     HIP_TRY(h, hipModuleGetFunction(&h->k_classify.rtc, h->module, "hpmc_classify_rtc"));
     HIP_TRY(h, hipModuleGetFunction(&h->k_emit.rtc, h->module, "hpmc_emit_rtc"));
     HIP_TRY(h, hipModuleGetFunction(&h->k_codes.rtc, h->module, "hpmc_codes_rtc"));
+    HIP_TRY(h, hipModuleGetFunction(&h->k_classify_small.rtc, h->module, "hpmc_classify_small_rtc"));
     return true;
 }
 
@@ -654,11 +656,13 @@ bool setup(HPMCHistoPyramid* h)
     if (h->mode == kFieldVolume) {
         if (h->unorm8) {
             h->k_classify.builtin = (const void*)hpmc_classify_volume8;
+            h->k_classify_small.builtin = (const void*)hpmc_classify_small_volume8;
             h->k_codes.builtin = (const void*)hpmc_codes_volume8;
             h->k_emit.builtin = h->binary_active ? (const void*)hpmc_emit_volume8_binary
                                                  : (const void*)hpmc_emit_volume8;
         } else if (h->gradient) {
             h->k_classify.builtin = (const void*)hpmc_classify_volume4;
+            h->k_classify_small.builtin = (const void*)hpmc_classify_small_volume4;
             h->k_codes.builtin = (const void*)hpmc_codes_volume4;
             h->k_emit.builtin = h->binary_active ? (const void*)hpmc_emit_volume4_binary
                                                  : (const void*)hpmc_emit_volume4_gradient;
@@ -668,6 +672,7 @@ bool setup(HPMCHistoPyramid* h)
             const uint64_t bytes = 4ull * (uint64_t)h->size[0] * (uint64_t)h->size[1] * (uint64_t)h->size[2];
             h->k_classify.builtin = bytes >= (256ull << 20) ? (const void*)hpmc_classify_volume_stream
                                                             : (const void*)hpmc_classify_volume;
+            h->k_classify_small.builtin = (const void*)hpmc_classify_small_volume;
             h->k_codes.builtin = (const void*)hpmc_codes_volume;
             h->k_emit.builtin = h->binary_active ? (const void*)hpmc_emit_volume_binary
                                                  : (const void*)hpmc_emit_volume;
@@ -748,7 +753,11 @@ bool launch_classify(HPMCHistoPyramid* h, float T)
     stage_mark(h, kStageClassify);
     // 8-bit volumes classify raw bytes against the equivalent byte threshold
     const float Tc = h->mode == kFieldVolume && h->unorm8 ? unorm8_threshold(T) : T;
-    HIP_TRY(h, hpmc_host::plan_classify(h->plan, h->k_classify, Tc, h->constants->d_cases, h->stream));
+    if (hpmc_host::plan_small_blocks(h->plan, h->plan.vols[0]))
+        HIP_TRY(h, hpmc_host::plan_classify_small(h->plan, h->k_classify_small, Tc, h->constants->d_cases,
+                                                  h->stream));
+    else
+        HIP_TRY(h, hpmc_host::plan_classify(h->plan, h->k_classify, Tc, h->constants->d_cases, h->stream));
     HIP_TRY(h, hipGetLastError());
     stage_mark(h, kStageClassify);
     return true;
@@ -1265,7 +1274,7 @@ int32_t HPMCgetCellCodes(struct HPMCHistoPyramid* h, uint8_t* codes)
 
 uint64_t HPMCgetLevel4(struct HPMCHistoPyramid* h, uint32_t* out, uint64_t capacity)
 {
-    if (!h || !out) {
+    if (!h) {
         fail(h, HPMC_ERR_NULL, "NULL argument");
         return 0;
     }
@@ -1274,6 +1283,10 @@ uint64_t HPMCgetLevel4(struct HPMCHistoPyramid* h, uint32_t* out, uint64_t capac
         return 0;
     }
     const uint64_t n = h->plan.vols[0].n_level4;
+    if (!out) {   // size query
+        ok(h);
+        return n;
+    }
     if (capacity < n) {
         fail(h, HPMC_ERR_CAPACITY, "need %llu entries", (unsigned long long)n);
         return 0;
@@ -1481,6 +1494,9 @@ int32_t HPMCbatchAddVolume(struct HPMCBatch* b, const float* volume, int32_t x, 
     std::string err;
     const int32_t rc = hpmc_host::layout_of(v.size, v.cells, &lay, &err);
     if (rc != HPMC_OK) return -fail(nullptr, rc, "%s", err.c_str());
+    // batched classification covers 16x16-cell blocks only
+    if (2 * lay.tile[0] < hpmc::kBlock || 2 * lay.tile[1] < hpmc::kBlock)
+        return -fail(nullptr, HPMC_ERR_UNSUPPORTED, "batched volumes need at least 9 cells along x and y");
     if (b->specs.size() >= 65535) return -fail(nullptr, HPMC_ERR_CAPACITY, "a batch holds at most 65535 volumes");
     b->specs.push_back(v);
     b->tainted = true;
@@ -1738,7 +1754,11 @@ int32_t HPMCextractVolumeNoAlloc(struct HPMCConstants* c, const float* volume, i
                                   nullptr};
     const uint32_t cap = (uint32_t)std::min<uint64_t>(capacity, 0xffffffffull);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hpmc_host::plan_classify(p, k_cls, threshold, c->d_cases, stream);
+    if (e == hipSuccess)
+        e = hpmc_host::plan_small_blocks(p, desc)
+                ? hpmc_host::plan_classify_small(p, hpmc_host::Kernel{(const void*)hpmc_classify_small_volume, nullptr},
+                                                 threshold, c->d_cases, stream)
+                : hpmc_host::plan_classify(p, k_cls, threshold, c->d_cases, stream);
     if (e == hipSuccess) e = hpmc_host::plan_scan(p, (const void*)hpmc_scan, stream);
     if (e == hipSuccess)
         e = hpmc_host::plan_emit(p, hpmc_host::Kernel{(const void*)hpmc_emit_volume, nullptr}, threshold, out,

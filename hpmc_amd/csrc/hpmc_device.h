@@ -55,7 +55,7 @@ struct Geom {
     int tsx2, tsy2;       // doubled tile sizes: cells per tile along x, y
     int tiles_x;          // slices per tile row
     int l2;               // log2 of the base-level side
-    int blocks_log2;      // log2 of the level-4 side (l2 + 1 - kBlockLog2)
+    int blocks_log2;      // log2 of the block level's side (l2 + 1 - blog2)
     float scx, scy, scz;  // extent / cells
     float ipx, ipy, ipz;  // 1 / lattice size (snippet texel-centre coordinates)
     int zseg;             // classification slices per workgroup: 8, 16 or 32
@@ -63,6 +63,11 @@ struct Geom {
     // tsx2, tsy2, tiles_x and zseg are powers of two: their log2, so that
     // key and slice arithmetic is shifts and masks
     int tsx2_log2, tsy2_log2, tiles_x_log2, zseg_log2;
+    // log2 of the block side: kBlockLog2 (16x16-cell blocks, HP level 4),
+    // or less when a doubled tile is narrower than 16 cells (grids of at most
+    // 8 cells along x or y): blocks never straddle two tiles, so each block
+    // is cells of one slice and its vertices are contiguous in the order
+    int blog2;
 };
 
 // Per (z segment, y, x) block column the vertex counts of its slices, 32
@@ -146,12 +151,13 @@ __device__ __forceinline__ uint32_t morton_compact(uint32_t v)
     return v;
 }
 
-// Level-4 block key of the 16x16 cell block starting at (x0, y0) of slice z.
+// Block key (Morton code of the block in the doubled base image) of the
+// block holding cell (x0, y0) of slice z.
 __device__ __forceinline__ uint32_t block_key(const Geom& g, int x0, int y0, int z)
 {
     const uint32_t tx = (uint32_t)z & ((1u << g.tiles_x_log2) - 1u), ty = (uint32_t)z >> g.tiles_x_log2;
-    const uint32_t X = ((tx << g.tsx2_log2) + (uint32_t)x0) >> kBlockLog2;
-    const uint32_t Y = ((ty << g.tsy2_log2) + (uint32_t)y0) >> kBlockLog2;
+    const uint32_t X = ((tx << g.tsx2_log2) + (uint32_t)x0) >> g.blog2;
+    const uint32_t Y = ((ty << g.tsy2_log2) + (uint32_t)y0) >> g.blog2;
     return morton_spread(X) | (morton_spread(Y) << 1);
 }
 
@@ -159,11 +165,11 @@ __device__ __forceinline__ uint32_t block_key(const Geom& g, int x0, int y0, int
 // grid), read from the column-major count array.
 __device__ __forceinline__ uint32_t key_count(const Geom& g, const uint16_t* __restrict__ colcnt, uint32_t key)
 {
-    const int X = (int)morton_compact(key) << kBlockLog2;
-    const int Y = (int)morton_compact(key >> 1) << kBlockLog2;
+    const int X = (int)morton_compact(key) << g.blog2;
+    const int Y = (int)morton_compact(key >> 1) << g.blog2;
     const int tx = X >> g.tsx2_log2, ty = Y >> g.tsy2_log2;
     const int z = (ty << g.tiles_x_log2) + tx;
-    const int bx = (X & (g.tsx2 - 1)) >> kBlockLog2, by = (Y & (g.tsy2 - 1)) >> kBlockLog2;
+    const int bx = (X & (g.tsx2 - 1)) >> g.blog2, by = (Y & (g.tsy2 - 1)) >> g.blog2;
     if (tx >= g.tiles_x || z >= g.cz || bx >= g.nbx || by >= g.nby) return 0u;
     const int pz = z >> g.zseg_log2, zi = z & (g.zseg - 1);
     return colcnt[(((size_t)pz * g.nby + by) * g.nbx + bx) * kColStride + zi];
@@ -668,7 +674,55 @@ __device__ __forceinline__ void classify_body(const F& field, const Geom& g, flo
     }
 }
 
-// ---------------------------------------------------------------------------
+// Classification of grids whose blocks are narrower than 16 cells (at most
+// 8 cells along x or y): one thread per column-count entry (z segment, block
+// row, block column, slice slot) adds up the table counts of the block's
+// cells in that slice, writes the entry (zero for slots past the grid) and
+// adds it into the scan tile sums.  Samples come from the field's plane-wise
+// fetch, which is what the classification of larger grids compares too
+// (raw bytes for 8-bit volumes, classified against the byte threshold).
+template <class F>
+__device__ __forceinline__ void classify_small_body(const F& field, const Geom& g, float T,
+                                                    uint16_t* __restrict__ colcnt, uint32_t n_cc,
+                                                    const uint8_t* __restrict__ lut, uint2* __restrict__ tsum_l4,
+                                                    uint2* __restrict__ tsum_col)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_cc) return;
+    const int zi = (int)(t % (uint32_t)kColStride);
+    const uint32_t w = t / (uint32_t)kColStride;
+    const int bx = (int)(w % (uint32_t)g.nbx);
+    const uint32_t rest = w / (uint32_t)g.nbx;
+    const int by = (int)(rest % (uint32_t)g.nby);
+    const int pz = (int)(rest / (uint32_t)g.nby);
+    const int z = pz * g.zseg + zi;
+    const int bs = 1 << g.blog2;
+    uint32_t cnt = 0;
+    if (zi < g.zseg && z < g.cz) {
+        const typename F::Plane p0 = field.plane(z), p1 = field.plane(z + 1);
+        const int x1 = min((bx + 1) * bs, g.cx), y1 = min((by + 1) * bs, g.cy);
+        for (int y = by * bs; y < y1; ++y)
+            for (int x = bx * bs; x < x1; ++x) {
+                uint32_t code = 0;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const uint32_t idx = field.lane_index(x + (b & 1), y + ((b >> 1) & 1));
+                    const float v = field.fetch((b & 4) ? p1 : p0, idx);
+                    code |= (v < T ? 1u : 0u) << b;
+                }
+                cnt += lut[code];
+            }
+    }
+    colcnt[t] = (uint16_t)cnt;
+    if (cnt) {
+        atomicAdd(&tsum_col[t / (uint32_t)kScanTile].x, cnt);
+        atomicAdd(&tsum_col[t / (uint32_t)kScanTile].y, chunks_of(cnt));
+        const uint32_t tile = block_key(g, bx * bs, by * bs, z) / (uint32_t)kScanTile;
+        atomicAdd(&tsum_l4[tile].x, cnt);
+        atomicAdd(&tsum_l4[tile].y, 1u);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // scan helpers
 
@@ -856,7 +910,7 @@ __device__ __forceinline__ void scan_body(
         for (int k = 0; k < kScanPer; ++k) {
             if (!c[k]) continue;
             const int zi = (int)(base % (uint32_t)kColStride) + k;
-            const uint32_t key = block_key(g, bxb << kBlockLog2, by << kBlockLog2, pz * g.zseg + zi);
+            const uint32_t key = block_key(g, bxb << g.blog2, by << g.blog2, pz * g.zseg + zi);
             const uint32_t yv = (c[k] << 4) | (vol << 16);
             for (uint32_t ch = 0; ch * kEmitChunk < c[k]; ++ch) items[off.y++] = make_uint2(key, yv | ch);
         }
@@ -960,8 +1014,8 @@ struct EmitShared {
 // Block origin (x0, y0, z) of a level-4 key.
 __device__ __forceinline__ void block_origin(const Geom& g, uint32_t key, int& x0, int& y0, int& z)
 {
-    const int X0 = (int)morton_compact(key) << kBlockLog2;
-    const int Y0 = (int)morton_compact(key >> 1) << kBlockLog2;
+    const int X0 = (int)morton_compact(key) << g.blog2;
+    const int Y0 = (int)morton_compact(key >> 1) << g.blog2;
     const int tx = X0 >> g.tsx2_log2, ty = Y0 >> g.tsy2_log2;
     z = (ty << g.tiles_x_log2) + tx;
     x0 = X0 & (g.tsx2 - 1);
@@ -1105,7 +1159,10 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
         const float* P0 = ws.plane[s0];
         const float* P1 = ws.plane[s1];
         if (!reuse) {
-            const bool qvx0 = cx0 + qx < g.cx, qvx1 = cx0 + qx + 1 < g.cx;
+            // cells of the block (blocks narrower than 16 take the corner
+            // of the 16x16 quad layout; quads are whole inside or outside)
+            const bool qin = qx < (1 << g.blog2) && qy < (1 << g.blog2);
+            const bool qvx0 = qin && cx0 + qx < g.cx, qvx1 = qin && cx0 + qx + 1 < g.cx;
             const bool qvy0 = cy0 + qy < g.cy, qvy1 = cy0 + qy + 1 < g.cy;
 
             // classify the quad: corners (qx..qx+2, qy..qy+2) of planes z, z+1
@@ -1662,6 +1719,17 @@ __device__ __forceinline__ void for_each_vertex(const TraversalView& v, const MK
         hpmc::classify_body(MAKE_FIELD, dd.g, T, colcnt + dd.colcnt_off, dd.g.zseg, lut, xch, \
                             stage, (int)(t - dd.cls_wg0), red, tile_sums + dd.l4_tile0,       \
                             tile_sums + n_l4t + dd.col_tile0);                                \
+    }                                                                                         \
+    extern "C" __global__ __launch_bounds__(256) void hpmc_classify_small_##SUFFIX(            \
+        const hpmc::VolDesc* vols, float T, uint16_t* colcnt, const uint64_t* cases,            \
+        uint2* tile_sums, uint32_t n_l4t)                                                     \
+    {                                                                                         \
+        __shared__ uint8_t lut[256];                                                          \
+        const hpmc::VolDesc dd = vols[0];                                                     \
+        lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);                              \
+        __syncthreads();                                                                      \
+        hpmc::classify_small_body(MAKE_FIELD, dd.g, T, colcnt + dd.colcnt_off, dd.n_cc, lut,  \
+                                  tile_sums + dd.l4_tile0, tile_sums + n_l4t + dd.col_tile0);  \
     }                                                                                         \
     extern "C" __global__ __launch_bounds__(256) void hpmc_codes_##SUFFIX(                     \
         const hpmc::VolDesc* vols, uint32_t vol, float T, uint8_t* codes)                     \

@@ -76,11 +76,6 @@ inline int32_t layout_of(const int32_t size[3], const int32_t cells[3], Layout* 
     out->tiles[0] = S / ts0;
     out->tiles[1] = S / ts1;
     out->l2 = l2;
-    // level-4 blocks are 16x16 cells of the doubled tiles
-    if (2 * ts0 < hpmc::kBlock || 2 * ts1 < hpmc::kBlock) {
-        *err = "grids with 8 or fewer cells along x or y are not supported";
-        return HPMC_ERR_UNSUPPORTED;
-    }
     return HPMC_OK;
 }
 
@@ -128,7 +123,6 @@ inline hpmc::Geom make_geom(const VolumeSpec& v, const Layout& lay, int zseg)
     g.tsy2 = 2 * lay.tile[1];
     g.tiles_x = lay.tiles[0];
     g.l2 = lay.l2;
-    g.blocks_log2 = lay.l2 + 1 - hpmc::kBlockLog2;
     g.scx = v.extent[0] / (float)v.cells[0];
     g.scy = v.extent[1] / (float)v.cells[1];
     g.scz = v.extent[2] / (float)v.cells[2];
@@ -136,9 +130,6 @@ inline hpmc::Geom make_geom(const VolumeSpec& v, const Layout& lay, int zseg)
     g.ipy = 1.0f / (float)v.size[1];
     g.ipz = 1.0f / (float)v.size[2];
     g.zseg = zseg;
-    g.nbx = (g.cx + hpmc::kBlock - 1) / hpmc::kBlock;
-    g.nby = (g.cy + hpmc::kBlock - 1) / hpmc::kBlock;
-    g.npz = (g.cz + zseg - 1) / zseg;
     auto log2_of = [](int v) {
         int l = 0;
         while ((1 << l) < v) ++l;
@@ -148,6 +139,14 @@ inline hpmc::Geom make_geom(const VolumeSpec& v, const Layout& lay, int zseg)
     g.tsy2_log2 = log2_of(g.tsy2);
     g.tiles_x_log2 = log2_of(g.tiles_x);
     g.zseg_log2 = log2_of(zseg);
+    // blocks of 16x16 cells (HP level 4), or the narrower doubled tile's
+    // side when a tile is below 16 cells: a block is always cells of one slice
+    g.blog2 = std::min(hpmc::kBlockLog2, std::min(g.tsx2_log2, g.tsy2_log2));
+    g.blocks_log2 = lay.l2 + 1 - g.blog2;
+    const int bs = 1 << g.blog2;
+    g.nbx = (g.cx + bs - 1) / bs;
+    g.nby = (g.cy + bs - 1) / bs;
+    g.npz = (g.cz + zseg - 1) / zseg;
     return g;
 }
 
@@ -306,8 +305,10 @@ inline int32_t plan_describe(Plan& p, const VolumeSpec* specs, size_t n, Layout*
             *err = "build too large for 32-bit sample keys";
             return HPMC_ERR_SIZE;
         }
-        // a block holds at most 256 * 15 vertices: 15 work items per block slice
-        p.n_items_max += (uint64_t)d.g.nbx * d.g.nby * d.g.cz * 15;
+        // a block of b x b cells holds at most 15 b^2 vertices: 15 work
+        // items per slice of a 16x16 block, fewer for narrower blocks
+        const uint64_t bcells = (uint64_t)1 << (2 * d.g.blog2);
+        p.n_items_max += (uint64_t)d.g.nbx * d.g.nby * d.g.cz * ((15 * bcells + hpmc::kEmitChunk - 1) / hpmc::kEmitChunk);
     }
 
     // persistent emit grid: as many 4-wave workgroups as fit at once, a
@@ -431,6 +432,28 @@ inline hipError_t plan_classify(const Plan& p, const Kernel& k, float T, const u
     // the XCD-aware workgroup mapping needs the grid padded to a multiple of 8
     const unsigned grid = (unsigned)(((uint64_t)p.n_wg + 7) / 8 * 8);
     return launch(k, grid, 64u * (unsigned)p.cls_waves, args, stream);
+}
+
+// Classification of a one-volume plan whose blocks are narrower than 16
+// cells (classify_small_body): one thread per column-count entry.
+inline hipError_t plan_classify_small(const Plan& p, const Kernel& k, float T, const uint64_t* cases,
+                                      hipStream_t stream)
+{
+    const hpmc::VolDesc* vols = p.d_vols;
+    float Ta = T;
+    uint16_t* cc = p.colcnt;
+    const uint64_t* cs = cases;
+    uint2* ts = p.tile_sums;
+    uint32_t nl = p.n_l4t;
+    void* args[] = {&vols, &Ta, &cc, &cs, &ts, &nl};
+    const unsigned grid = (unsigned)((p.n_cc + 255) / 256);
+    return launch(k, grid, 256, args, stream);
+}
+
+// True when the plan's (single) volume has blocks narrower than 16 cells.
+inline bool plan_small_blocks(const Plan& p, const hpmc::VolDesc& d)
+{
+    return p.nvol == 1 && d.g.blog2 < hpmc::kBlockLog2;
 }
 
 inline hipError_t plan_scan(const Plan& p, const void* k_scan, hipStream_t stream)

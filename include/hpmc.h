@@ -228,9 +228,12 @@ uint32_t HPMCacquireDroppedVertices(struct HPMCTraversalHandle* th);
  * bytes.  Enqueued. */
 int32_t HPMCgetCellCodes(struct HPMCHistoPyramid* h, uint8_t* device_codes);
 
-/* Copies the pyramid level holding 16x16-cell sums (level 4 of the
- * reference's base-level pyramid), indexed by Morton block code, into a
- * device buffer; returns the entry count, or 0 on failure. */
+/* Copies the pyramid level holding the block sums of the last build,
+ * indexed by Morton block code, into a device buffer; returns the entry
+ * count, or 0 on failure (with device_out NULL: the count only).  Blocks are
+ * 16x16 cells of the doubled base image (level 4 of the reference's
+ * base-level pyramid), or b x b cells (level log2 b) for grids whose doubled
+ * tile is narrower, b its side (grids of at most 8 cells along x or y). */
 uint64_t HPMCgetLevel4(struct HPMCHistoPyramid* h, uint32_t* device_out, uint64_t capacity);
 
 /* Per-stage timing with HIP events on the HistoPyramid's stream: when

@@ -80,6 +80,7 @@ def lib():
         L.hpo_vertex_list.restype = i64
         L.hpo_vertex_list.argtypes = [vp, vp, vp, vp, vp, i64]
         L.hpo_level4.argtypes = [vp, vp, vp, vp]
+        L.hpo_level_blocks.argtypes = [vp, vp, vp, ctypes.c_int, vp]
         L.hpo_emit_f64.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, ctypes.c_float, vp, vp, vp, i64, vp, vp, vp]
         L.hpo_emit_f32.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, ctypes.c_float, vp, vp, vp, i64, vp]
         L.hpo_run_volume.restype = i64
@@ -140,11 +141,23 @@ def vertex_list(counts, cells, lay, literal=False):
     return cell[:n], slot[:n]
 
 
-def level4(counts, cells, lay):
-    """16x16-cell block sums of the base image, indexed by Morton block code."""
-    side = (2 << lay[4]) // 16
+def block_log2(lay):
+    """log2 of the block side the HIP build sums: 4 (16x16 cells, HP level
+    4), or the smaller log2 of a doubled tile side (grids of at most 8 cells
+    along x or y), so that no block straddles two tiles."""
+    ts = [int(lay[0]) * 2, int(lay[1]) * 2]
+    return min(4, *(int(t).bit_length() - 1 for t in ts))
+
+
+def level4(counts, cells, lay, b=None):
+    """Block sums of the base image (2^b x 2^b cells of the doubled image, b
+    = block_log2(lay) by default: HP level 4 for all but the narrowest
+    grids), indexed by Morton block code."""
+    b = block_log2(lay) if b is None else b
+    side = (2 << lay[4]) >> b
     out = np.zeros(side * side, np.uint32)
-    lib().hpo_level4(_ptr(np.ascontiguousarray(counts, np.uint8)), _ptr(_i32(cells)), _ptr(_i32(lay)), _ptr(out))
+    lib().hpo_level_blocks(_ptr(np.ascontiguousarray(counts, np.uint8)), _ptr(_i32(cells)), _ptr(_i32(lay)), int(b),
+                           _ptr(out))
     return out
 
 

@@ -258,20 +258,25 @@ int64_t hpo_vertex_list(const uint8_t* counts, const int32_t cells[3], const int
     return n;
 }
 
-/* Level-4 sums of the pyramid (16x16-cell blocks of the doubled base image, */
-/* the channels of HP level 4), indexed by the Morton code of the block.      */
-/* out has (2S/16)^2 entries.                                                 */
-void hpo_level4(const uint8_t* counts, const int32_t cells[3], const int32_t lay[5], uint32_t* out)
+/* Block sums of the pyramid: 2^b x 2^b-cell blocks of the doubled base     */
+/* image (the channels of HP level b; level 4 = 16x16 cells), indexed by the */
+/* Morton code of the block.  out has (2S / 2^b)^2 entries.                  */
+void hpo_level_blocks(const uint8_t* counts, const int32_t cells[3], const int32_t lay[5], int b, uint32_t* out)
 {
-    int64_t side = ((int64_t)2 << lay[4]) / 16;
+    int64_t side = ((int64_t)2 << lay[4]) >> b;
     memset(out, 0, (size_t)(side * side) * sizeof(uint32_t));
     for (int64_t z = 0; z < cells[2]; ++z)
         for (int64_t y = 0; y < cells[1]; ++y)
             for (int64_t x = 0; x < cells[0]; ++x) {
                 uint32_t X, Y;
                 cell_image_pos(lay, x, y, z, &X, &Y);
-                out[morton2(X >> 4, Y >> 4)] += counts[(z * cells[1] + y) * cells[0] + x];
+                out[morton2(X >> b, Y >> b)] += counts[(z * cells[1] + y) * cells[0] + x];
             }
+}
+
+void hpo_level4(const uint8_t* counts, const int32_t cells[3], const int32_t lay[5], uint32_t* out)
+{
+    hpo_level_blocks(counts, cells, lay, 4, out);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -16,6 +16,7 @@ int64_t hpo_hp_traverse(const uint8_t* counts, const int32_t cells[3], const int
 int64_t hpo_vertex_list(const uint8_t* counts, const int32_t cells[3], const int32_t lay[5],
                         uint32_t* out_cell, uint8_t* out_slot, int64_t cap);
 void hpo_level4(const uint8_t* counts, const int32_t cells[3], const int32_t lay[5], uint32_t* out);
+void hpo_level_blocks(const uint8_t* counts, const int32_t cells[3], const int32_t lay[5], int b, uint32_t* out);
 void hpo_emit_f64(const float* f, const int32_t dims[3], const int32_t size[3], int clamp,
                   const int32_t cells[3], const float ext[3], float T,
                   const uint8_t* codes, const uint32_t* cell, const uint8_t* slot, int64_t n,

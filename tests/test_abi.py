@@ -160,7 +160,8 @@ def _histopyramid(lattice, grid=None):
 
 
 @pytest.mark.parametrize("lattice", [(64, 64, 64), (256, 256, 256), (512, 512, 512), (33, 20, 45),
-                                     (150, 70, 90), (17, 300, 5), (1000, 10, 200), (16384, 16, 3)])
+                                     (150, 70, 90), (17, 300, 5), (1000, 10, 200), (16384, 16, 3),
+                                     (3, 3, 3), (5, 9, 4), (9, 3, 20), (4, 40, 40), (6, 6, 6)])
 def test_layout_query_matches_oracle(lattice):
     # init.cpp:105-130 in FP32, host only
     L, c, h = _histopyramid(lattice)
@@ -173,15 +174,14 @@ def test_layout_query_matches_oracle(lattice):
 
 
 @pytest.mark.parametrize("lattice,grid", [((1, 8, 8), None), ((8, 8, 16385), None), ((8, 8, 8), (8, 7, 7)),
-                                          ((8, 8, 8), (7, 0, 7)), ((8, 2, 8), None), ((6, 6, 6), None)])
+                                          ((8, 8, 8), (7, 0, 7)), ((8, 2, 8), None), ((2, 9, 9), None)])
 def test_layout_rejects_bad_sizes(lattice, grid):
-    # init.cpp:82-103: lattice in [2, 16384], 1 <= grid < lattice; this build
-    # also needs more than 8 cells along x and y (16-cell blocks of the
-    # doubled tiles) and at least 2 (the reference's log2f(0.5) shift)
+    # init.cpp:82-103: lattice in [2, 16384], 1 <= grid < lattice; at least 2
+    # cells along x and y (one cell gives the reference's log2f(0.5) shift)
     L, c, h = _histopyramid(lattice, grid)
     out = (ctypes.c_int32 * 5)()
     code = L.HPMCgetLayout(h, out)
-    assert code in (3, 9), code           # HPMC_ERR_SIZE / HPMC_ERR_UNSUPPORTED
+    assert code == 3, code                # HPMC_ERR_SIZE
     assert L.HPMCgetLastError(h) == code
     L.HPMCdestroyHandle(h)
     L.HPMCdestroyConstants(c)

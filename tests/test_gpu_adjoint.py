@@ -43,7 +43,7 @@ def _rel(a, b):
 
 # 48^3 = 110,592 samples: 17 key bits, sorted with 9-bit digits (two passes)
 @pytest.mark.parametrize("shape,ext", [((40, 40, 40), (1.0, 1.0, 1.0)), ((37, 52, 29), (2.0, 0.5, 1.0)),
-                                       ((48, 48, 48), (1.0, 1.0, 1.0))])
+                                       ((48, 48, 48), (1.0, 1.0, 1.0)), ((8, 60, 60), (1.0, 1.0, 1.0))])
 def test_adjoint_matches_fp64_oracle(shape, ext):
     f = sphere_noise(shape, 7)
     T = 0.0

@@ -36,6 +36,54 @@ def test_volume_extreme_shapes_bit_exact(lattice):
     assert _check(res, f, 0.5, lattice) > 100
 
 
+# Grids of 2 to 8 cells along x or y (the reference takes any lattice of 2 to
+# 16384 samples per axis, init.cpp:82-103): doubled tiles narrower than 16
+# cells, so blocks are 2x2, 4x4 or 8x8 cells and several tiles share a
+# 16x16 square of the doubled image; order and sums are those of the
+# reference's pyramid all the same.
+TINY = [(3, 3, 3), (5, 9, 4), (9, 3, 20), (4, 40, 40), (6, 6, 6), (3, 17, 50), (40, 5, 33), (8, 8, 300),
+        (130, 6, 12), (7, 9, 2)]
+
+
+def _tiny_field(lattice, seed):
+    # white noise: every small lattice gets many crossings
+    rng = np.random.default_rng(seed)
+    return rng.random(lattice[::-1]).astype(np.float32)
+
+
+@pytest.mark.parametrize("lattice", TINY)
+def test_volume_tiny_grids_bit_exact(lattice):
+    f = _tiny_field(lattice, sum(lattice))
+    res = _gpu(lattice, 0.5, volume=f)
+    assert _check(res, f, 0.5, lattice) > 0
+
+
+@pytest.mark.parametrize("lattice", [(5, 9, 4), (4, 40, 40), (3, 17, 50)])
+def test_tiny_grids_other_fields(lattice):
+    # snippet, 8-bit, gradient and binary fields through the same blocks
+    from parity_util import with_gradient
+    res = _gpu(lattice, 0.0, snippet=workloads.CAYLEY_SNIPPET)
+    _check(res, workloads.cayley(lattice, extra=2), 0.0, lattice, clamp=False)
+    f = _tiny_field(lattice, 7)
+    b = np.clip(np.rint(f * 255.0), 0, 255).astype(np.uint8)
+    res8 = _gpu(lattice, 0.5, volume=b)
+    ref = _gpu(lattice, 0.5, volume=(b.astype(np.float32) / np.float32(255.0)).astype(np.float32))
+    assert res8["n"] == ref["n"] > 0 and np.array_equal(res8["ce"], ref["ce"])
+    assert np.array_equal(res8["verts"], ref["verts"])
+    g = with_gradient(f)
+    resg = _gpu(lattice, 0.5, volume=g, gradient=True)
+    assert _check(resg, f, 0.5, lattice, mode="gradient", field4=g) > 0
+    resb = _gpu(lattice, 0.5, volume=f, binary=True)
+    assert _check(resb, f, 0.5, lattice, mode="binary") > 0
+
+
+def test_tiny_grid_smaller_than_lattice_and_extent():
+    lattice, grid, ext = (12, 30, 20), (6, 21, 17), (0.5, 2.0, 1.0)
+    f = _tiny_field(lattice, 13)
+    res = _gpu(lattice, 0.5, volume=f, grid=grid, ext=ext)
+    assert _check(res, f, 0.5, lattice, grid=grid, ext=ext) > 0
+
+
 def test_volume_grid_smaller_than_lattice_and_extent():
     lattice, grid, ext = (48, 40, 36), (30, 25, 33), (2.0, 0.5, 1.5)
     f = _smooth_noise(lattice, 11)

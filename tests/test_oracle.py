@@ -421,3 +421,29 @@ def test_adjoint_dT_finite_difference():
 
     fd = (4.0 * fd_T(h / 2) - fd_T(h)) / 3.0
     assert abs(fd - dT) <= 2e-6 * max(1.0, abs(fd)), (fd, dT)
+
+
+@pytest.mark.parametrize("lattice", [(5, 9, 4), (9, 3, 20), (4, 40, 40), (33, 17, 6)])
+def test_block_sums_match_dense_doubled_image(lattice):
+    # block sums of any block side 2^b equal the sums of a dense doubled
+    # base image (each cell placed at (2 tile_x ts_x + x, 2 tile_y ts_y + y)
+    # by numpy) over 2^b x 2^b squares, read in Morton order
+    rng = np.random.default_rng(sum(lattice))
+    cells = tuple(v - 1 for v in lattice)
+    counts = rng.integers(0, 16, size=cells[::-1]).astype(np.uint8)
+    lay = oracle.layout(lattice, cells)
+    W = 2 << lay[4]
+    img = np.zeros((W, W), np.int64)
+    z, y, x = np.meshgrid(*[np.arange(c) for c in cells[::-1]], indexing="ij")
+    X = (z % lay[2]) * 2 * lay[0] + x
+    Y = (z // lay[2]) * 2 * lay[1] + y
+    np.add.at(img, (Y.ravel(), X.ravel()), counts.ravel().astype(np.int64))
+    for b in range(1, 5):
+        side = W >> b
+        sums = img.reshape(side, 1 << b, side, 1 << b).sum(axis=(1, 3))     # [Y block, X block]
+        bx, by = np.meshgrid(np.arange(side), np.arange(side))
+        keys = np.array([oracle.lib().hpo_morton2(int(i), int(j)) for i, j in zip(bx.ravel(), by.ravel())])
+        want = np.zeros(side * side, np.int64)
+        want[keys] = sums[by.ravel(), bx.ravel()]
+        assert oracle.level4(counts, cells, lay, b).tolist() == want.tolist(), b
+    assert oracle.block_log2(lay) == min(4, int(2 * lay[0]).bit_length() - 1, int(2 * lay[1]).bit_length() - 1)

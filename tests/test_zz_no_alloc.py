@@ -45,6 +45,9 @@ def test_no_alloc_equals_build_and_extract_on_shrinking_calls():
         ("cayley 160x144x128", lambda: torch.from_numpy(workloads.cayley((160, 144, 128))).cuda(), 0.0,
          (1.0, 1.0, 1.0)),
         ("noise 45x38x33", lambda: torch.from_numpy(smooth_noise((45, 38, 33), 3)).cuda(), 0.5, (2.0, 1.0, 0.5)),
+        ("white noise 5x9x40 (4x4-cell blocks)",
+         lambda: torch.from_numpy(np.random.default_rng(1).random((40, 9, 5)).astype(np.float32)).cuda(), 0.5,
+         (1.0, 1.0, 1.0)),
     ]
     for name, make, T, ext in cases:
         print("no-alloc case:", name, flush=True)
