@@ -50,9 +50,19 @@ def cayley_volume_torch(n, device):
     return (4.0 * s + xyz - 1.0).contiguous()
 
 
+def host_cpus():
+    """CPUs this process may run on (the host's share), and the machine's."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return usable, os.cpu_count() or usable
+
+
 def cpu_baseline(n, reps):
     """Oracle C port of classify -> ordered list -> FP32 emission, one thread,
-    over the whole 512^3 workload, median of `reps` runs."""
+    over the whole 512^3 workload, median of `reps` runs.  The line states the
+    host's CPUs beside the one thread used."""
     import oracle
     from hpmc_amd import workloads
     f = workloads.cayley((n, n, n))
@@ -64,8 +74,45 @@ def cpu_baseline(n, reps):
         times.append(time.perf_counter() - t0)
     t = sorted(times)[len(times) // 2]
     cells = (n - 1) ** 3
+    usable, machine = host_cpus()
     return {"value": cells / t, "unit": "cells/s", "cores": 1, "kind": "port",
+            "host_cpus_usable": usable, "host_cpus": machine,
             "sample": "full %d^3 Cayley volume, %d runs (median %.2f s), %d vertices" % (n, reps, t, nv)}
+
+
+def cpu_baseline_threads(n, reps, threads=None):
+    """The same port over z slabs on `threads` host threads (default: the
+    usable CPUs, at most 16): each thread classifies its slab, lists its
+    vertices in the full volume's order and emits them (ctypes releases the
+    GIL).  The slabs' lists are not concatenated into the global order, so
+    this is a throughput line beside the one-thread baseline, not a drop-in
+    result."""
+    import concurrent.futures
+    import numpy as np
+    import oracle
+    from hpmc_amd import workloads
+    usable, machine = host_cpus()
+    threads = threads or max(1, min(16, usable))
+    f = workloads.cayley((n, n, n))
+    cz = n - 1
+    bounds = [(i * cz // threads, (i + 1) * cz // threads) for i in range(threads)]
+    oracle.lib()
+
+    def slab(b):
+        return oracle.run_volume(f, 0.0, z_begin=b[0], z_end=b[1], emit_out=True)
+
+    times = []
+    nv = 0
+    with concurrent.futures.ThreadPoolExecutor(threads) as pool:
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            nv = sum(pool.map(slab, bounds))
+            times.append(time.perf_counter() - t0)
+    t = sorted(times)[len(times) // 2]
+    return {"value": cz ** 3 / t, "unit": "cells/s", "cores": threads, "kind": "port",
+            "host_cpus_usable": usable, "host_cpus": machine,
+            "sample": "full %d^3 Cayley volume in %d z slabs on %d threads, %d runs (median %.3f s), %d vertices"
+                      % (n, threads, threads, reps, t, int(np.int64(nv)))}
 
 
 def measure_batched(reps, count=256, iso=1.0):
@@ -561,6 +608,7 @@ def main():
         extra("adjoint", lambda: measure_adjoint(args.batch_reps))
     if not args.no_cpu_baseline:
         extra("cpu_baseline", lambda: cpu_baseline(n, args.cpu_reps))
+        extra("cpu_baseline_threads", lambda: cpu_baseline_threads(n, args.cpu_reps))
     print(json.dumps(line))
 
 

@@ -15,7 +15,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhpmc_amd.so")
+# HPMC_AMD_LIBRARY names another build of the library (a sanitizer build of
+# the host code, for instance); the default is the in-tree libhpmc_amd.so
+LIB_PATH = os.environ.get("HPMC_AMD_LIBRARY") or os.path.join(HERE, "libhpmc_amd.so")
 
 HPMC_OK = 0
 ERRORS = {

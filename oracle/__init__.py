@@ -154,7 +154,7 @@ def level4(counts, cells, lay, b=None):
     = block_log2(lay) by default: HP level 4 for all but the narrowest
     grids), indexed by Morton block code."""
     b = block_log2(lay) if b is None else b
-    side = (2 << lay[4]) >> b
+    side = max((2 << lay[4]) >> b, 1)
     out = np.zeros(side * side, np.uint32)
     lib().hpo_level_blocks(_ptr(np.ascontiguousarray(counts, np.uint8)), _ptr(_i32(cells)), _ptr(_i32(lay)), int(b),
                            _ptr(out))

@@ -260,10 +260,12 @@ int64_t hpo_vertex_list(const uint8_t* counts, const int32_t cells[3], const int
 
 /* Block sums of the pyramid: 2^b x 2^b-cell blocks of the doubled base     */
 /* image (the channels of HP level b; level 4 = 16x16 cells), indexed by the */
-/* Morton code of the block.  out has (2S / 2^b)^2 entries.                  */
+/* Morton code of the block.  out has side^2 entries, side = max(1, 2S/2^b) */
+/* (one block covers a doubled image narrower than 2^b).                     */
 void hpo_level_blocks(const uint8_t* counts, const int32_t cells[3], const int32_t lay[5], int b, uint32_t* out)
 {
     int64_t side = ((int64_t)2 << lay[4]) >> b;
+    if (side < 1) side = 1;
     memset(out, 0, (size_t)(side * side) * sizeof(uint32_t));
     for (int64_t z = 0; z < cells[2]; ++z)
         for (int64_t y = 0; y < cells[1]; ++y)
