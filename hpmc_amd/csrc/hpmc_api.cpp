@@ -21,8 +21,6 @@
 #include <string>
 #include <vector>
 
-#include <rocprim/device/device_radix_sort.hpp>
-
 #include "hpmc.h"
 #include "hpmc_device.h"
 
@@ -64,23 +62,63 @@ extern "C" __global__ __launch_bounds__(256) void hpmc_level4_gather(const hpmc:
     hpmc::level4_body(d, colcnt, out);
 }
 
-HPMC_DEFINE_ADJOINT_KERNEL(hpmc_adjoint_volume, HPMC_VOLUME_FIELD, false)
-HPMC_DEFINE_ADJOINT_KERNEL(hpmc_adjoint_volume_batch, HPMC_VOLUME_FIELD, true)
-
-extern "C" __global__ __launch_bounds__(256) void hpmc_adjoint_segments(const uint32_t* keys, const float* vals,
-                                                                        uint32_t n, float* grad)
+// Adjoint passes (hpmc_device.h, adjoint_cells_body and following).
+extern "C" __global__ __launch_bounds__(256) void hpmc_adjoint_cells(const hpmc::VolDesc* vols, float T,
+                                                                     const uint2* items, const uint32_t* l4_off,
+                                                                     const hpmc::Totals* totals,
+                                                                     const uint64_t* cases, const uint32_t* tile0,
+                                                                     uint8_t* codes, uint32_t* cell_off,
+                                                                     uint8_t* flags)
 {
-    hpmc::adjoint_segments_body(keys, vals, n, grad);
+    __shared__ uint8_t lut[256];
+    lut[threadIdx.x] = (uint8_t)(cases[threadIdx.x] & 0xfu);
+    __syncthreads();
+    hpmc::adjoint_cells_body(vols, T, items, l4_off, totals, lut, tile0, codes, cell_off, flags);
 }
 
-extern "C" __global__ __launch_bounds__(256) void hpmc_sum_partials(const float* x, uint32_t n, float* part)
+extern "C" __global__ __launch_bounds__(256) void hpmc_adjoint_records(const hpmc::VolDesc* vols, float T,
+                                                                       const uint2* items,
+                                                                       const hpmc::Totals* totals,
+                                                                       const uint64_t* cases, const uint8_t* codes,
+                                                                       const uint32_t* cell_off, const float* gout,
+                                                                       uint32_t nvert, float* rec, uint64_t stride,
+                                                                       float* partials)
 {
-    hpmc::sum_partials_body(x, n, part);
+    __shared__ uint16_t smask[256][12];
+    __shared__ uint16_t lists[hpmc::kAdjointWaves][hpmc::kEdgeListMax];
+    hpmc::fill_slot_masks(cases, smask);
+    __syncthreads();
+    hpmc::adjoint_records_body(vols, T, items, totals, smask, codes, cell_off, gout, nvert, rec, stride, partials,
+                               lists[threadIdx.x >> 6]);
 }
 
-extern "C" __global__ __launch_bounds__(256) void hpmc_sum_final(const float* part, float* out)
+extern "C" __global__ __launch_bounds__(1024) void hpmc_adjoint_tile_list(const uint8_t* flags, uint32_t ntiles,
+                                                                           uint32_t* list, uint32_t* count)
 {
-    hpmc::sum_final_body(part, out);
+    hpmc::adjoint_tile_list_body(flags, ntiles, list, count);
+}
+
+extern "C" __global__ __launch_bounds__(256) void hpmc_adjoint_gather(const hpmc::VolDesc* vols,
+                                                                      const uint32_t* tile_vol,
+                                                                      const uint32_t* tile0, const uint32_t* list,
+                                                                      const uint32_t* count, const uint8_t* codes,
+                                                                      const uint32_t* cell_off, const float* rec,
+                                                                      uint64_t stride, uint32_t nvert, float* grad)
+{
+    __shared__ uint32_t tab[3][hpmc::kGatherOrigins];
+    hpmc::adjoint_gather_body(vols, tile_vol, tile0, list, count, codes, cell_off, rec, stride, nvert, grad, tab);
+}
+
+extern "C" __global__ __launch_bounds__(1024) void hpmc_adjoint_sum_items(const float* partials,
+                                                                          const hpmc::Totals* totals, float* out)
+{
+    hpmc::adjoint_sum_items_body(partials, totals, out);
+}
+
+extern "C" __global__ __launch_bounds__(256) void hpmc_fill_u32(uint32_t* p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 extern "C" __global__ __launch_bounds__(256) void hpmc_fill_f32(float* p, uint64_t n, float v)
@@ -107,28 +145,26 @@ extern "C" __global__ __launch_bounds__(256) void hpmc_noalloc_begin(hpmc::VolDe
 
 namespace hpmc_host {
 
-// Scratch of the adjoint, grown on demand: 7 (key, value) pairs per vertex
-// twice (sort input and output), per-vertex dL/dT terms, partial sums and
-// the radix sort's temporary storage.
+// Scratch of the adjoint, grown on demand: per lattice sample a cell code
+// and a first vertex (cells by their origin sample), followed by one flag
+// per gather tile; per vertex room for one edge record; per work item a
+// dL/dT partial sum.
 struct AdjointScratch {
-    uint64_t cap_pairs = 0;
-    uint32_t* keys[2] = {nullptr, nullptr};
-    float* vals[2] = {nullptr, nullptr};
-    float* dT = nullptr;
+    uint64_t cap_vertices = 0, cap_lattice = 0, cap_items = 0, cap_tiles = 0;
+    uint8_t* codes = nullptr;
+    uint32_t* cell_off = nullptr;
+    float* rec = nullptr;
     float* partials = nullptr;
-    void* sort_tmp = nullptr;
-    size_t sort_tmp_bytes = 0;
+    uint32_t* tiles = nullptr;      // flagged gather tiles, then their count
 };
 
 inline void adjoint_free(AdjointScratch& s)
 {
-    for (int i = 0; i < 2; ++i) {
-        if (s.keys[i]) (void)hipFree(s.keys[i]);
-        if (s.vals[i]) (void)hipFree(s.vals[i]);
-    }
-    if (s.dT) (void)hipFree(s.dT);
+    if (s.codes) (void)hipFree(s.codes);
+    if (s.cell_off) (void)hipFree(s.cell_off);
+    if (s.rec) (void)hipFree(s.rec);
     if (s.partials) (void)hipFree(s.partials);
-    if (s.sort_tmp) (void)hipFree(s.sort_tmp);
+    if (s.tiles) (void)hipFree(s.tiles);
     s = AdjointScratch();
 }
 
@@ -143,84 +179,78 @@ inline void adjoint_free(AdjointScratch& s)
 
 // dL/dfield (n_lattice floats, every sample written) and dL/dT (one float)
 // of the last extraction of plan p (nvert vertices at threshold T), from
-// dL/dvertices gout (nvert x 6, normal then position).  Deterministic:
-// per-vertex contributions, a stable sort by sample, in-order run sums.
-// rocPRIM's onesweep radix sort with 9-bit digits, for key widths where that
-// needs one pass fewer than the 8-bit digits of its default configuration
-// for 4-byte keys and values on gfx950 (1024-thread blocks, 8 items per
-// thread, match ranking): 17-18 and 25-27 bits (81M pairs on 26 bits: 2.4 ->
-// 1.55 ms).  A stable sort has one result, so the choice changes no bits.
-using SortCfg9 = rocprim::radix_sort_config<
-    rocprim::default_config, rocprim::default_config,
-    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 8>, rocprim::kernel_config<1024, 8>, 9,
-                                        rocprim::block_radix_rank_algorithm::match>>;
-
-inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, const Kernel& k, float T, uint32_t nvert,
-                           const float* gout, float* grad_field, float* grad_iso, const uint64_t* cases,
-                           hipStream_t stream, std::string* err)
+// dL/dvertices gout (nvert x 6, normal then position).  Deterministic, with
+// no float atomics and no sorting: per-cell codes and first vertices from
+// the HistoPyramid's blocks, one record per sign-changing lattice edge
+// formed by the cell that owns it, a per-sample gather of the records in a
+// fixed order, and a fixed-shape sum of the per-item dL/dT terms.
+inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, float T, uint32_t nvert, const float* gout,
+                           float* grad_field, float* grad_iso, const uint64_t* cases, hipStream_t stream,
+                           std::string* err)
 {
-    const uint64_t npairs = (uint64_t)hpmc::kAdjointPairs * nvert;
-    if (npairs > 0xffffffffull) {
-        *err = "too many vertices for the adjoint";
+    // 32-bit byte offsets into the record planes and a volume's first-vertex
+    // array
+    if (nvert >= (1u << 30)) {
+        *err = "too many vertices for the adjoint (at most 2^30 - 1)";
         return HPMC_ERR_SIZE;
     }
-    if (npairs > s.cap_pairs || !s.partials) {
-        ADJ_TRY(hipStreamSynchronize(stream));
-        adjoint_free(s);
-        const uint64_t cap = std::max<uint64_t>(npairs, hpmc::kAdjointPairs);
-        for (int i = 0; i < 2; ++i) {
-            ADJ_TRY(hipMalloc(&s.keys[i], cap * sizeof(uint32_t)));
-            ADJ_TRY(hipMalloc(&s.vals[i], cap * sizeof(float)));
+    for (const hpmc::VolDesc& d : p.vols) {
+        if ((uint64_t)d.g.sx * d.g.sy * d.g.sz > (1ull << 30)) {
+            *err = "volume too large for the adjoint (at most 2^30 samples)";
+            return HPMC_ERR_SIZE;
         }
-        ADJ_TRY(hipMalloc(&s.dT, cap / hpmc::kAdjointPairs * sizeof(float)));
-        ADJ_TRY(hipMalloc(&s.partials, hpmc::kReduceBlocks * sizeof(float)));
-        size_t bytes8 = 0, bytes9 = 0;
-        ADJ_TRY(rocprim::radix_sort_pairs(nullptr, bytes8, s.keys[0], s.keys[1], s.vals[0], s.vals[1],
-                                          (unsigned)cap, 0, 32, stream));
-        ADJ_TRY(rocprim::radix_sort_pairs<SortCfg9>(nullptr, bytes9, s.keys[0], s.keys[1], s.vals[0], s.vals[1],
-                                                    (unsigned)cap, 0, 32, stream));
-        s.sort_tmp_bytes = std::max(bytes8, bytes9);
-        ADJ_TRY(hipMalloc(&s.sort_tmp, std::max<size_t>(s.sort_tmp_bytes, 16)));
-        s.cap_pairs = cap;
     }
-    hipLaunchKernelGGL(hpmc_fill_f32, dim3(1024), dim3(256), 0, stream, grad_field, (uint64_t)p.n_lattice, 0.0f);
-    ADJ_TRY(hipGetLastError());
+    const uint64_t need_v = std::max<uint64_t>(nvert, 1), need_l = std::max<uint64_t>(p.n_lattice, 1);
+    const uint64_t need_i = std::max<uint64_t>(p.n_items_max, 1), need_t = std::max<uint64_t>(p.n_gather_tiles, 1);
+    if (need_v > s.cap_vertices || need_l > s.cap_lattice || need_i > s.cap_items || need_t > s.cap_tiles) {
+        ADJ_TRY(hipStreamSynchronize(stream));
+        const uint64_t cv = std::max(need_v, s.cap_vertices), cl = std::max(need_l, s.cap_lattice);
+        const uint64_t ci = std::max(need_i, s.cap_items), ct = std::max(need_t, s.cap_tiles);
+        adjoint_free(s);
+        ADJ_TRY(hipMalloc(&s.codes, (cl + 15) / 16 * 16 + (ct + 15) / 16 * 16));
+        ADJ_TRY(hipMalloc(&s.cell_off, cl * sizeof(uint32_t)));
+        ADJ_TRY(hipMalloc(&s.rec, cv * hpmc::kAdjointRecord * sizeof(float)));
+        ADJ_TRY(hipMalloc(&s.partials, ci * sizeof(float)));
+        ADJ_TRY(hipMalloc(&s.tiles, (ct + 1) * sizeof(uint32_t)));
+        s.cap_vertices = cv;
+        s.cap_lattice = cl;
+        s.cap_items = ci;
+        s.cap_tiles = ct;
+    }
     if (nvert == 0) {
+        hipLaunchKernelGGL(hpmc_fill_f32, dim3(1024), dim3(256), 0, stream, grad_field, (uint64_t)p.n_lattice, 0.0f);
         hipLaunchKernelGGL(hpmc_fill_f32, dim3(1), dim3(64), 0, stream, grad_iso, (uint64_t)1, 0.0f);
         ADJ_TRY(hipGetLastError());
         return HPMC_OK;
     }
-    {
-        const hpmc::VolDesc* vols = p.d_vols;
-        float Ta = T;
-        const uint2* it = p.items;
-        const uint32_t* lo = p.l4_off;
-        const hpmc::Totals* tt = p.d_totals;
-        uint32_t cnt = nvert;
-        const uint64_t* cs = cases;
-        const float* go = gout;
-        uint32_t* ko = s.keys[0];
-        float* vo = s.vals[0];
-        float* dt = s.dT;
-        void* args[] = {&vols, &Ta, &it, &lo, &tt, &cnt, &cs, &go, &ko, &vo, &dt};
-        ADJ_TRY(launch(k, (unsigned)p.emit_wgs, 256, args, stream));
-    }
-    // keys < n_lattice <= 2^end_bit - 1: the sort looks at end_bit bits
-    int end_bit = 1;
-    while (end_bit < 32 && ((uint64_t)1 << end_bit) <= p.n_lattice) ++end_bit;
-    size_t tmp_bytes = s.sort_tmp_bytes;
-    if ((end_bit + 8) / 9 < (end_bit + 7) / 8) {
-        ADJ_TRY(rocprim::radix_sort_pairs<SortCfg9>(s.sort_tmp, tmp_bytes, s.keys[0], s.keys[1], s.vals[0],
-                                                    s.vals[1], (unsigned)npairs, 0, end_bit, stream));
-    } else {
-        ADJ_TRY(rocprim::radix_sort_pairs(s.sort_tmp, tmp_bytes, s.keys[0], s.keys[1], s.vals[0], s.vals[1],
-                                          (unsigned)npairs, 0, end_bit, stream));
-    }
-    hipLaunchKernelGGL(hpmc_adjoint_segments, dim3(2048), dim3(256), 0, stream, (const uint32_t*)s.keys[1],
-                       (const float*)s.vals[1], (uint32_t)npairs, grad_field);
-    hipLaunchKernelGGL(hpmc_sum_partials, dim3(hpmc::kReduceBlocks), dim3(256), 0, stream, (const float*)s.dT,
-                       nvert, s.partials);
-    hipLaunchKernelGGL(hpmc_sum_final, dim3(1), dim3(256), 0, stream, (const float*)s.partials, grad_iso);
+    // codes (n_lattice bytes) and the gather tile flags after them, each
+    // rounded up to 16 bytes, cleared together
+    uint8_t* flags = s.codes + (p.n_lattice + 15) / 16 * 16;
+    const uint64_t clear_words = (p.n_lattice + 15) / 16 * 4 + (p.n_gather_tiles + 15) / 16 * 4;
+    hipLaunchKernelGGL(hpmc_fill_u32, dim3((unsigned)std::min<uint64_t>((clear_words + 255) / 256, 2048)), dim3(256), 0,
+                       stream, (uint32_t*)s.codes, clear_words, 0u);
+    const unsigned item_grid =
+        (unsigned)std::min<uint64_t>((p.n_items_max + hpmc::kAdjointWaves - 1) / hpmc::kAdjointWaves, 4096);
+    hipLaunchKernelGGL(hpmc_adjoint_cells, dim3(item_grid), dim3(64 * hpmc::kAdjointWaves), 0, stream,
+                       (const hpmc::VolDesc*)p.d_vols, T, (const uint2*)p.items, (const uint32_t*)p.l4_off,
+                       (const hpmc::Totals*)p.d_totals, cases, (const uint32_t*)p.d_gather_tile0, s.codes, s.cell_off,
+                       flags);
+    hipLaunchKernelGGL(hpmc_adjoint_records, dim3(item_grid), dim3(64 * hpmc::kAdjointWaves), 0, stream,
+                       (const hpmc::VolDesc*)p.d_vols, T, (const uint2*)p.items, (const hpmc::Totals*)p.d_totals,
+                       cases, (const uint8_t*)s.codes, (const uint32_t*)s.cell_off, gout, nvert, s.rec,
+                       (uint64_t)s.cap_vertices, s.partials);
+    // dL/dfield: zero, then the flagged tiles gathered
+    hipLaunchKernelGGL(hpmc_fill_f32, dim3(4096), dim3(256), 0, stream, grad_field, (uint64_t)p.n_lattice, 0.0f);
+    uint32_t* count = s.tiles + p.n_gather_tiles;
+    hipLaunchKernelGGL(hpmc_adjoint_tile_list, dim3(1), dim3(1024), 0, stream, (const uint8_t*)flags,
+                       (uint32_t)p.n_gather_tiles, s.tiles, count);
+    hipLaunchKernelGGL(hpmc_adjoint_gather, dim3((unsigned)std::min<uint64_t>(p.n_gather_tiles, 2048)),
+                       dim3(hpmc::kGatherThreads), 0, stream, (const hpmc::VolDesc*)p.d_vols,
+                       (const uint32_t*)p.d_gather_vol, (const uint32_t*)p.d_gather_tile0, (const uint32_t*)s.tiles,
+                       (const uint32_t*)count, (const uint8_t*)s.codes, (const uint32_t*)s.cell_off,
+                       (const float*)s.rec, (uint64_t)s.cap_vertices, nvert, grad_field);
+    hipLaunchKernelGGL(hpmc_adjoint_sum_items, dim3(1), dim3(1024), 0, stream, (const float*)s.partials,
+                       (const hpmc::Totals*)p.d_totals, grad_iso);
     ADJ_TRY(hipGetLastError());
     return HPMC_OK;
 }
@@ -1352,9 +1382,7 @@ int32_t HPMCadjoint(struct HPMCTraversalHandle* th, const float* grad_vertices, 
     if (!wait_count(h)) return h->last_error;
     if (h->count && !grad_vertices) return fail(h, HPMC_ERR_NULL, "grad_vertices == NULL");
     std::string err;
-    const int32_t rc = hpmc_host::adjoint_run(h->plan, h->adjoint,
-                                              hpmc_host::Kernel{(const void*)hpmc_adjoint_volume, nullptr},
-                                              h->threshold, h->count, grad_vertices, grad_field, grad_iso,
+    const int32_t rc = hpmc_host::adjoint_run(h->plan, h->adjoint, h->threshold, h->count, grad_vertices, grad_field, grad_iso,
                                               h->constants->d_cases, h->stream, &err);
     if (rc != HPMC_OK) return fail(h, rc, "%s", err.c_str());
     ok(h);
@@ -1640,9 +1668,7 @@ int32_t HPMCbatchAdjoint(struct HPMCBatch* b, const float* grad_vertices, float*
     const uint32_t n = b->plan.h_totals->vertices;
     if (n && !grad_vertices) return fail(nullptr, HPMC_ERR_NULL, "grad_vertices == NULL");
     std::string err;
-    const int32_t rc = hpmc_host::adjoint_run(b->plan, b->adjoint,
-                                              hpmc_host::Kernel{(const void*)hpmc_adjoint_volume_batch, nullptr},
-                                              b->last_threshold, n, grad_vertices, grad_field, grad_iso,
+    const int32_t rc = hpmc_host::adjoint_run(b->plan, b->adjoint, b->last_threshold, n, grad_vertices, grad_field, grad_iso,
                                               b->constants->d_cases, b->stream, &err);
     if (rc != HPMC_OK) return fail(nullptr, rc, "%s", err.c_str());
     ok(nullptr);

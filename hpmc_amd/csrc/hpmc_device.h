@@ -947,22 +947,7 @@ constexpr int kPlaneLoads = (kBrickPlane + 63) / 64;   // 6 per lane per plane
 //   continuous: t from the two edge samples, normal from forward differences
 //   binary:     edge midpoint, normal from the per-case table (threshold 0.5)
 //   gradient:   t from the two samples' values, normal from their gradients
-enum EmitMode { kEmitContinuous = 0, kEmitBinary = 1, kEmitGradient = 2, kEmitAdjoint = 3 };
-
-// Adjoint of continuous emission: per output vertex o, dL/d(vertex o) is
-// read from gout[6o..6o+5] (normal, position) and the vertex's
-// contributions to dL/dfield are written as kAdjointPairs (key, value)
-// pairs at keys/vals[7o..7o+6] (key = lattice_base + sample index) and its
-// contribution to dL/dT at dT[o].  kAdjointNoKey, above every sample index,
-// pads the run-sum kernel's tiles.
-constexpr int kAdjointPairs = 7;
-constexpr uint32_t kAdjointNoKey = 0xffffffffu;
-struct AdjointIO {
-    const float* gout;
-    uint32_t* keys;
-    float* vals;
-    float* dT;
-};
+enum EmitMode { kEmitContinuous = 0, kEmitBinary = 1, kEmitGradient = 2 };
 
 // Per-vertex callback of user-fused traversal kernels: emission calls
 // cb(k, normal, position) for output vertex k instead of writing the
@@ -990,7 +975,7 @@ struct wants_edge<CB, decltype((void)CB::kWantsEdge, void())> {
 // Lattice planes an emit item stages: z, z+1 for the corners, z+2 for the
 // forward differences of continuous mode.
 template <int MODE>
-constexpr int emit_planes() { return MODE == kEmitContinuous || MODE == kEmitAdjoint ? 3 : 2; }
+constexpr int emit_planes() { return MODE == kEmitContinuous ? 3 : 2; }
 
 // A staged plane holds the 18x18 window in kPlaneStride slots (6 loads of 64
 // lanes, no tail); planes sit kPlaneStride floats apart so a sample and its
@@ -1071,8 +1056,7 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                                           const uint32_t nitems,
                                           float* __restrict__ out, CellEdge* __restrict__ out_ce,
                                           uint32_t capacity, const float4* __restrict__ bnorm, EmitShared& sh,
-                                          uint64_t case_word, const AdjointIO adj = AdjointIO{},
-                                          const CB& cb = CB{})
+                                          uint64_t case_word, const CB& cb = CB{})
 {
     constexpr int NP = emit_planes<MODE>();
     const int lane = threadIdx.x & 63;
@@ -1359,74 +1343,7 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
 
         // batches of 64 consecutive vertices, one per lane (lanes past the
         // chunk's end repeat its last vertex and store nothing)
-        if constexpr (MODE == kEmitAdjoint) {
-            for (uint32_t v0 = vlo; v0 < vhi; v0 += 64) {
-                const uint32_t v = min(v0 + (uint32_t)lane, vhi - 1u);
-                const uint32_t o = base + v;
-                if (v0 + (uint32_t)lane >= vhi || o >= capacity) continue;
-                const Site s = site_of(v);
-                const EdgeSamples q = samples_of(s);
-                const int axis = s.axis;
-                // p = sc (a + t axis), n = sc (T - (na (1-t) + nb t)),
-                // t = (va - T) / (va - vb): chain rule back to the samples
-                // va, vb, na_q = f(a + e_q), nb_q = f(b + e_q) (na_axis is
-                // vb's sample) and to T
-                const float tt = (q.va - T) * __builtin_amdgcn_rcpf(q.va - q.vb);
-                const float u = 1.0f - tt;
-                const float2* gi = (const float2*)(adj.gout + (size_t)o * 6);
-                const float2 g01 = gi[0], g23 = gi[1], g45 = gi[2];
-                const float wx = g01.x * g.scx, wy = g01.y * g.scy, wz = g23.x * g.scz;
-                float gt = axis == 0 ? g23.y * g.scx : (axis == 1 ? g45.x * g.scy : g45.y * g.scz);
-                gt += wx * (q.nax - q.nbx) + wy * (q.nay - q.nby) + wz * (q.naz - q.nbz);
-                const float rD = __builtin_amdgcn_rcpf(q.va - q.vb);
-                const float dva = gt * (T - q.vb) * rD * rD;
-                const float dvb = gt * (q.va - T) * rD * rD;
-                const float dT = (wx + wy + wz) - gt * rD;
-                const float dnax = -wx * u, dnay = -wy * u, dnaz = -wz * u;
-                const auto field = mk(dc);
-                const int Ax = cx0 + s.ax, Ay = cy0 + s.ay, Az = cz + s.az;
-                const int Bx = Ax + (axis == 0), By = Ay + (axis == 1), Bz = Az + (axis == 2);
-                const uint32_t lb = dc.lattice_base;
-                uint4 k0;
-                uint3 k1;
-                float4 w0;
-                float3 w1;
-                k0.x = lb + field.index(Ax, Ay, Az);
-                w0.x = dva;
-                k0.y = lb + field.index(Bx, By, Bz);
-                w0.y = dvb + (axis == 0 ? dnax : (axis == 1 ? dnay : dnaz));
-                // the two non-axis forward samples of a
-                const int q0 = axis == 0 ? 1 : 0, q1 = axis == 2 ? 1 : 2;
-                k0.z = lb + field.index(Ax + (q0 == 0), Ay + (q0 == 1), Az);
-                w0.z = q0 == 0 ? dnax : dnay;
-                k0.w = lb + field.index(Ax, Ay + (q1 == 1), Az + (q1 == 2));
-                w0.w = q1 == 1 ? dnay : dnaz;
-                k1.x = lb + field.index(Bx + 1, By, Bz);
-                w1.x = -wx * tt;
-                k1.y = lb + field.index(Bx, By + 1, Bz);
-                w1.y = -wy * tt;
-                k1.z = lb + field.index(Bx, By, Bz + 1);
-                w1.z = -wz * tt;
-                // seven pairs per vertex: 4-byte aligned dword stores
-                uint32_t* kd = adj.keys + (size_t)o * kAdjointPairs;
-                float* vd = adj.vals + (size_t)o * kAdjointPairs;
-                kd[0] = k0.x;
-                kd[1] = k0.y;
-                kd[2] = k0.z;
-                kd[3] = k0.w;
-                kd[4] = k1.x;
-                kd[5] = k1.y;
-                kd[6] = k1.z;
-                vd[0] = w0.x;
-                vd[1] = w0.y;
-                vd[2] = w0.z;
-                vd[3] = w0.w;
-                vd[4] = w1.x;
-                vd[5] = w1.y;
-                vd[6] = w1.z;
-                adj.dT[o] = dT;
-            }
-        } else if constexpr (CB::kActive) {
+        if constexpr (CB::kActive) {
             // a user-fused traversal: hand each vertex over, write nothing
             for (uint32_t v0 = vlo; v0 < vhi; v0 += 64) {
                 const uint32_t v = min(v0 + (uint32_t)lane, vhi - 1u);
@@ -1464,177 +1381,641 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     }
 }
 
-// Adjoint reduction, after the (key, value) pairs are sorted by key with
-// ties in vertex order: every run of equal keys is summed from 0 in array
-// order, one addition after the other, so the result does not depend on the
-// launch.  A wave takes a tile of kSegTile pairs, loaded coalesced into LDS
-// and read back 16 consecutive ones per lane.  Each lane walks its 16 pairs
-// once: runs that start and end in them are written at once; the run its
-// pairs end with, if it goes on into the next lane, leaves its running sum
-// there as a carry.  A lane whose 16 pairs all continue the run of the lane
-// before passes the carry on after adding its own (a few rounds at most: a
-// sample feeds at most a few dozen vertices), and the lane where the run
-// ends adds its head pairs to the carry and writes the sum.  Runs arriving
-// from the previous tile belong to it; a run going on past the tile is
-// finished from the next 64 pairs, staged in LDS (and from global memory in
-// the unlikely case that it is longer still).  Launched with 256 threads per
-// workgroup.
-constexpr int kSegPer = 16;
-constexpr int kSegTile = 64 * kSegPer;
-// x of lane - 1 (lane0 in lane 0) and of lane + 1 (lane63 in lane 63): DPP
-// wave_shr:1 / wave_shl:1
-__device__ __forceinline__ uint32_t wave_shr1_or(uint32_t x, uint32_t lane0)
-{
-    const uint32_t y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x138, 0xf, 0xf, false);   // wave_shr:1
-    return (threadIdx.x & 63) == 0 ? lane0 : y;
-}
-__device__ __forceinline__ uint32_t wave_shl1_or(uint32_t x, uint32_t lane63)
-{
-    const uint32_t y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x130, 0xf, 0xf, false);   // wave_shl:1
-    return (threadIdx.x & 63) == 63 ? lane63 : y;
-}
-__device__ __forceinline__ void adjoint_segments_body(const uint32_t* __restrict__ keys,
-                                                      const float* __restrict__ vals, uint32_t n,
-                                                      float* __restrict__ grad)
-{
-    // per wave: its tile and the 64 pairs after it, one pad word per 16
-    // pairs so that the lanes reading 16 consecutive pairs hit different banks
-    constexpr int kStaged = kSegTile + 64;
-    __shared__ uint32_t sk[4][kStaged + kStaged / 16];
-    __shared__ float sv[4][kStaged + kStaged / 16];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
-    const uint64_t ntiles = ((uint64_t)n + kSegTile - 1) / kSegTile;
-    for (uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + w; t < ntiles; t += nwaves) {
-        const uint64_t base = t * kSegTile;
-        // coalesced loads, all in flight at once (indices clamped into the
-        // array; pairs past its end read as the unused key), staged in LDS
-        constexpr int kLoads = kStaged / 64;
-        uint32_t lk[kLoads];
-        float lv[kLoads];
-#pragma unroll
-        for (int k = 0; k < kLoads; ++k) {
-            const uint64_t i = min(base + 64 * k + lane, (uint64_t)n - 1);
-            lk[k] = keys[i];
-            lv[k] = vals[i];
-        }
-        const uint32_t kprev = base > 0 ? keys[base - 1] : kAdjointNoKey;
-#pragma unroll
-        for (int k = 0; k < kLoads; ++k) {
-            const int e = 64 * k + lane;
-            sk[w][e + (e >> 4)] = base + e < n ? lk[k] : kAdjointNoKey;
-            sv[w][e + (e >> 4)] = lv[k];
-        }
-        wave_lds_sync();
-        uint32_t kk[kSegPer];
-        float vv[kSegPer];
-#pragma unroll
-        for (int i = 0; i < kSegPer; ++i) {
-            kk[i] = sk[w][17 * lane + i];   // pair 16 lane + i
-            vv[i] = sv[w][17 * lane + i];
-        }
-        const uint32_t ak0 = sk[w][kSegTile + (kSegTile >> 4)];   // the first pair after the tile
+// ---------------------------------------------------------------------------
+// Adjoint of continuous emission: dL/dfield and dL/dT of the last build's
+// vertices from dL/dvertex (6 floats per vertex, normal then position),
+// deterministic, with no atomics and no sorting.  The vertices on one
+// lattice edge (repeated by the triangles of the up to four cells around
+// it) share position and normal, so the adjoint is formed per edge from the
+// sum of their dL/dvertex.  For the edge from a to b = a + e_axis,
+// t = (va - T) / (va - vb),
+//   p = sc (a + t e_axis),  n = sc (T - (na (1 - t) + nb t)),
+// with na_q = f(a + e_q), nb_q = f(b + e_q) (na_axis is vb), so with
+// D = va - vb: dt/dva = (T - vb) / D^2, dt/dvb = (va - T) / D^2,
+// dt/dT = -1 / D; samples past the lattice read (and receive) the clamped
+// sample, as the forward reads them.
+//
+// Three passes:
+//  cells    one wave per non-empty block (the first work item of each):
+//           the cells' codes and each non-empty cell's first output vertex
+//           (the block offset plus the Morton-order prefix of the counts,
+//           as emission numbers them), into arrays indexed like the lattice
+//           samples (codes cleared beforehand), and a flag for every gather
+//           tile the block's edges can reach;
+//  records  one wave per non-empty block: the sign-changing edges its cells
+//           own, listed in LDS and spread over the lanes; per edge the
+//           summed dL/dvertex of its vertices (cells in a fixed order,
+//           slots in order), the seven sample contributions and the dL/dT
+//           term, one record stored at the owner's first vertex + the
+//           edge's rank among the owner's edges (every sign-changing edge
+//           of a case carries a slot, so a cell owns no more edges than it
+//           has vertices);
+//  gather   one workgroup per tile of samples: every sample sums the
+//           contributions that reach it in a fixed (axis, kind) order and
+//           is written, zero where none does (unflagged tiles at once).
+// Edge (a, axis) lies in the cells a - j e_p - k e_q (j, k in {0, 1}, p < q
+// the other axes); its owner is the first of those inside the grid in the
+// order (0,0), (1,0), (0,1), (1,1): a cell owns the three edges from its
+// origin and, on the grid's upper faces, the edges there.
+constexpr int kAdjointRecord = 7;           // contributions per edge record, one plane each
+constexpr uint32_t kNoEdge = 0xffffffffu;
+constexpr int kAdjointWaves = 4;            // waves per workgroup of the cells and records passes
+constexpr int kEdgeListMax = 1024;          // listed edges of a block (more: each lane takes its own)
 
-        // the key before this lane's pairs and the one after them (for the
-        // last lane, the first staged pair after the tile)
-        const uint32_t before = wave_shr1_or(kk[kSegPer - 1], kprev);
-        const uint32_t next = wave_shl1_or(kk[0], ak0);
-        // walk: runs that start here are summed; `head` while the pairs still
-        // continue the run of `before`
-        uint32_t run = before, nhead = kSegPer;
-        bool head = true;
-        float sum = 0.0f;
+// Gather tiles of 16 x 8 x 8 samples; the edges reaching a tile start at
+// most two samples below it, their owners at most three cells below.
+constexpr int kGatherX = 16, kGatherY = 8, kGatherZ = 8;
+constexpr int kGatherOX = kGatherX + 2, kGatherOY = kGatherY + 2, kGatherOZ = kGatherZ + 2;
+constexpr int kGatherOrigins = kGatherOX * kGatherOY * kGatherOZ;
+constexpr int kGatherCX = kGatherX + 3, kGatherCY = kGatherY + 3, kGatherCZ = kGatherZ + 3;
+constexpr int kGatherCells = kGatherCX * kGatherCY * kGatherCZ;
+constexpr int kGatherThreads = 256;
+constexpr int kGatherCellLoads = (kGatherCells + kGatherThreads - 1) / kGatherThreads;
+constexpr int kGatherOriginLoads = (kGatherOrigins + kGatherThreads - 1) / kGatherThreads;
+static_assert(kGatherX * kGatherY * kGatherZ == 4 * kGatherThreads, "four samples a thread");
+
+// Sign-changing edges of a cell code (bit e for edge e = 4 axis + s0 + 2 s1;
+// corner bit x + 2y + 4z).
+__device__ __forceinline__ uint32_t code_edges(uint32_t code)
+{
+    const uint32_t dx = code ^ (code >> 1), dy = code ^ (code >> 2), dz = code ^ (code >> 4);
+    return (dx & 1u) | ((dx >> 1) & 2u) | ((dx >> 2) & 4u) | ((dx >> 3) & 8u) | ((dy & 3u) << 4) |
+           (((dy >> 4) & 3u) << 6) | ((dz & 0xfu) << 8);
+}
+
+// Edges that cell (x, y, z) owns: all but those on its upper faces, unless
+// the face is the grid's.
+__device__ __forceinline__ uint32_t owned_edges(const Geom& g, int x, int y, int z)
+{
+    uint32_t m = 0xfffu;
+    if (x < g.cx - 1) m &= ~0xaa0u;   // x + 1: y and z edges with s0 = 1
+    if (y < g.cy - 1) m &= ~0xc0au;   // y + 1: x edges with s0 = 1, z edges with s1 = 1
+    if (z < g.cz - 1) m &= ~0x0ccu;   // z + 1: x and y edges with s1 = 1
+    return m;
+}
+
+// Per case and edge, the slots of the case on the edge (smask[code][e],
+// LDS), from the 64-bit case words (count, then 4-bit edges); one thread
+// per case.
+__device__ __forceinline__ void fill_slot_masks(const uint64_t* __restrict__ cases, uint16_t (*smask)[12])
+{
+    const uint64_t w = cases[threadIdx.x];
+    uint32_t m[12] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+    const int cnt = (int)(w & 0xfu);
+    for (int sl = 0; sl < cnt; ++sl) {
+        const uint32_t e = (uint32_t)(w >> (4 + 4 * sl)) & 0xfu;
 #pragma unroll
-        for (int i = 0; i < kSegPer; ++i) {
-            if (kk[i] != run) {
-                if (!head && run != kAdjointNoKey) grad[run] = sum;
-                if (head) nhead = i;
-                head = false;
-                run = kk[i];
-                sum = 0.0f;
+        for (int k = 0; k < 12; ++k) m[k] |= (e == (uint32_t)k) ? (1u << sl) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) smask[threadIdx.x][k] = (uint16_t)m[k];
+}
+
+// A buffer resource over `bytes` bytes at base: raw loads through it take
+// 32-bit byte offsets and read 0 past the end.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* base, uint64_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)(uint32_t)min(bytes, (uint64_t)0xffffffffu),
+                                             0x00020000);
+}
+__device__ __forceinline__ uint32_t buffer_load_u32(__amdgpu_buffer_rsrc_t r, uint32_t byte_offset)
+{
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, byte_offset, 0, 0);
+}
+__device__ __forceinline__ uint32_t buffer_load_u8(__amdgpu_buffer_rsrc_t r, uint32_t byte_offset)
+{
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, byte_offset, 0, 0);
+}
+
+// A volume's samples as a global-memory pointer: loads through it are
+// global loads (counted only by vmcnt), not flat ones, whose waits would
+// also cover LDS traffic.
+typedef const __attribute__((address_space(1))) float* GlobalFloats;
+__device__ __forceinline__ GlobalFloats global_field(const VolDesc& d) { return (GlobalFloats)d.field; }
+
+// Linear index of an in-range sample (or of the cell with that origin).
+__device__ __forceinline__ uint32_t lattice_index(const Geom& g, int x, int y, int z)
+{
+    return ((uint32_t)z * (uint32_t)g.sy + (uint32_t)y) * (uint32_t)g.sx + (uint32_t)x;
+}
+
+// Sum over the wave in a fixed order (butterfly).
+__device__ __forceinline__ float wave_sum_f(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// cells pass.  Lane l takes the 2x2 quad of cells with Morton indices
+// 4l .. 4l+3 of the block; lut = per-code vertex counts (LDS); tile0[v] =
+// first gather tile of volume v; flags: one byte per gather tile.
+__device__ __forceinline__ void adjoint_cells_body(const VolDesc* __restrict__ vols, float T,
+                                                   const uint2* __restrict__ items,
+                                                   const uint32_t* __restrict__ l4_off,
+                                                   const Totals* __restrict__ totals, const uint8_t* lut,
+                                                   const uint32_t* __restrict__ tile0, uint8_t* __restrict__ codes,
+                                                   uint32_t* __restrict__ cell_off, uint8_t* __restrict__ flags)
+{
+    const uint32_t nitems = load_uniform(&totals->items);
+    const int lane = threadIdx.x & 63;
+    const uint32_t w0 = blockIdx.x * kAdjointWaves + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kAdjointWaves;
+    const int qx = (int)morton_compact((uint32_t)lane) * 2, qy = (int)morton_compact((uint32_t)lane >> 1) * 2;
+    for (uint32_t it = w0; it < nitems; it += nw) {
+        const uint2 item = load_uniform(items + it);
+        if (item.y & 0xfu) continue;   // a further chunk of a block
+        const uint32_t vol = item_volume(item.y);
+        const VolDesc d = load_uniform(vols + vol);
+        const Geom& g = d.g;
+        const uint32_t obase = load_uniform(l4_off + d.l4_base + item.x);
+        const uint32_t t0 = load_uniform(tile0 + vol);
+        int x0, y0, z;
+        block_origin(g, item.x, x0, y0, z);
+        const int side = 1 << g.blog2;
+        // the quad's 3x3 corners on planes z and z + 1 (clamped indices)
+        const VolumeField f{(const float*)d.field, g.sx, g.sy, g.sz};
+        const GlobalFloats fv = global_field(d);
+        float v[2][3][3];
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) v[k][dy][dx] = fv[f.index(x0 + qx + dx, y0 + qy + dy, z + k)];
+        uint32_t code[4], cnt[4], cnt4 = 0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int ox = q & 1, oy = q >> 1;
+            const bool valid = qx < side && qy < side && x0 + qx + ox < g.cx && y0 + qy + oy < g.cy;
+            uint32_t c = 0u;
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                    for (int cx = 0; cx < 2; ++cx)
+                        c |= (v[k][oy + cy][ox + cx] < T ? 1u : 0u) << (cx + 2 * cy + 4 * k);
+            code[q] = c;
+            cnt[q] = valid ? (uint32_t)lut[c] : 0u;
+            cnt4 += cnt[q];
+        }
+        uint32_t ex = wave_scan_u32<false>(cnt4) - cnt4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (cnt[q]) {
+                const uint32_t s = d.lattice_base + lattice_index(g, x0 + qx + (q & 1), y0 + qy + (q >> 1), z);
+                codes[s] = (uint8_t)code[q];
+                cell_off[s] = obase + ex;
             }
-            if (!head) sum += vv[i];
+            ex += cnt[q];
         }
-        const bool goes_on = run == next;
-        if (!head && !goes_on && run != kAdjointNoKey) grad[run] = sum;
-        // carries: `out` is the running sum of the run this lane ends with,
-        // `known` once it is, `owned` when that run started in this tile
-        float out = sum;
-        bool known = !head;
-        bool owned = !head;
-        while (__ballot(!known) != 0ull) {
-            const float cin = __builtin_bit_cast(float, wave_shr1_or(__builtin_bit_cast(uint32_t, out), 0u));
-            const bool kin = wave_shr1_or((uint32_t)known, 1u) != 0u;
-            const bool oin = wave_shr1_or((uint32_t)owned, 0u) != 0u;
-            if (!known && kin) {
-                float r = cin;
-#pragma unroll
-                for (int i = 0; i < kSegPer; ++i) r += vv[i];
-                out = r;
-                owned = oin;
-                known = true;
-            }
+        // gather tiles the block's edges reach: samples x0 .. x0 + side + 2
+        // (and y likewise), z .. z + 3
+        const int tx0 = x0 / kGatherX, tx1 = min(x0 + side + 2, g.sx - 1) / kGatherX;
+        const int ty0 = y0 / kGatherY, ty1 = min(y0 + side + 2, g.sy - 1) / kGatherY;
+        const int tz0 = z / kGatherZ, tz1 = min(z + 3, g.sz - 1) / kGatherZ;
+        const int nx = tx1 - tx0 + 1, ny = ty1 - ty0 + 1, nz = tz1 - tz0 + 1;
+        if (lane < nx * ny * nz) {
+            const int ntx = (g.sx + kGatherX - 1) / kGatherX, nty = (g.sy + kGatherY - 1) / kGatherY;
+            const int tx = tx0 + lane % nx, ty = ty0 + (lane / nx) % ny, tz = tz0 + lane / (nx * ny);
+            flags[t0 + ((uint32_t)tz * nty + ty) * ntx + tx] = 1;
         }
-        const float cin = __builtin_bit_cast(float, wave_shr1_or(__builtin_bit_cast(uint32_t, out), 0u));
-        const bool oin = wave_shr1_or((uint32_t)owned, 0u) != 0u;
-        // the run arriving from the lane before ends in this lane's head
-        if (oin && before != kAdjointNoKey && nhead > 0 && nhead < kSegPer) {
-            float r = cin;
-#pragma unroll
-            for (int i = 0; i < kSegPer; ++i)
-                if (i < (int)nhead) r += vv[i];
-            grad[before] = r;
-        }
-        // ... or exactly at its last pair
-        if (head && owned && !goes_on && run != kAdjointNoKey) grad[run] = out;
-        // the run going on past the tile (last lane)
-        if (lane == 63 && goes_on && owned && run != kAdjointNoKey) {
-            float r = out;
-            int j = 0;
-            for (; j < 64 && sk[w][kSegTile + j + ((kSegTile + j) >> 4)] == run; ++j)
-                r += sv[w][kSegTile + j + ((kSegTile + j) >> 4)];
-            if (j == 64)
-                for (uint64_t g = base + kSegTile + 64; g < n && keys[g] == run; ++g) r += vals[g];
-            grad[run] = r;
-        }
-        wave_lds_sync();
     }
 }
 
-// dL/dT: a fixed-shape reduction of the per-vertex terms (partials of
-// kReduceBlocks contiguous ranges, then one block over the partials), the
-// same for every launch.
-constexpr int kReduceBlocks = 2048;
-__device__ __forceinline__ float block_sum_f(float v, float* sh)
+// The record of edge e of cell (x, y, z) (an owned sign-changing edge);
+// returns its dL/dT term and stores contribution k at rec[k stride + ri]
+// unless ri is kNoEdge (records are stored as seven planes of stride floats,
+// so that a gather of one kind reads with 32-bit element offsets).  vcodes
+// and voffs: buffers over the volume's codes and first vertices; smask: the
+// slot masks of fill_slot_masks.
+// cases: the 256 case words (LDS).  Vertices at or past nvert (only if the
+// field changed since the build) are skipped, so every access stays inside
+// the vertex arrays.
+__device__ __forceinline__ float adjoint_edge_record(const VolDesc& d, float T, const uint16_t (*smask)[12],
+                                                     __amdgpu_buffer_rsrc_t vcodes, __amdgpu_buffer_rsrc_t voffs,
+                                                     const float* __restrict__ gout, uint32_t nvert, int x, int y,
+                                                     int z, int e, float* __restrict__ rec, size_t stride,
+                                                     uint32_t ri)
 {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const Geom& g = d.g;
+    const int axis = e >> 2, s0 = e & 1, s1 = (e >> 1) & 1;
+    // first end: axis x (0, s0, s1), axis y (s0, 0, s1), axis z (s0, s1, 0)
+    const int Ax = x + (axis == 0 ? 0 : s0);
+    const int Ay = y + (axis == 0 ? s0 : (axis == 1 ? 0 : s1));
+    const int Az = z + (axis == 2 ? 0 : s1);
+    const int Bx = Ax + (axis == 0), By = Ay + (axis == 1), Bz = Az + (axis == 2);
+    // loads that need nothing else first: the edge's eight samples and the
+    // code and first vertex of the cells around it, A - j e_p - k e_q in the
+    // order (0,0), (1,0), (0,1), (1,1) (cells outside the grid read past the
+    // buffers' ends: code 0, no slots)
+    const VolumeField f{(const float*)d.field, g.sx, g.sy, g.sz};
+    const GlobalFloats fv = global_field(d);
+    const float va = fv[f.index(Ax, Ay, Az)], vb = fv[f.index(Bx, By, Bz)];
+    const float nax = fv[f.index(Ax + 1, Ay, Az)], nay = fv[f.index(Ax, Ay + 1, Az)];
+    const float naz = fv[f.index(Ax, Ay, Az + 1)];
+    const float nbx = fv[f.index(Bx + 1, By, Bz)], nby = fv[f.index(Bx, By + 1, Bz)];
+    const float nbz = fv[f.index(Bx, By, Bz + 1)];
+    // the slots on the edge of the four cells as one 64-bit list, cell jk
+    // in bits 16 jk .. 16 jk + 14, and their first vertices
+    uint64_t slots = 0u;
+    uint32_t off[4], cc[4];
+#pragma unroll
+    for (int jk = 0; jk < 4; ++jk) {
+        const int j = jk & 1, k = jk >> 1;
+        const int cx = Ax - (axis == 0 ? 0 : j);
+        const int cy = Ay - (axis == 0 ? j : (axis == 2 ? k : 0));
+        const int cz = Az - (axis == 2 ? 0 : k);
+        const bool in = cx >= 0 && cy >= 0 && cz >= 0 && cx < g.cx && cy < g.cy && cz < g.cz;
+        const uint32_t sc = in ? lattice_index(g, cx, cy, cz) : kNoEdge;
+        cc[jk] = buffer_load_u8(vcodes, sc);
+        off[jk] = buffer_load_u32(voffs, sc * 4u);
+    }
+#pragma unroll
+    for (int jk = 0; jk < 4; ++jk)
+        slots |= (uint64_t)smask[cc[jk]][4 * axis + (jk & 1) + 2 * (jk >> 1)] << (16 * jk);
+    // dL/dvertex summed over the edge's vertices in list order (cells in
+    // order, each cell's slots in order): the first kPreload loaded together,
+    // any further ones after
+    constexpr int kPreload = 4;
+    auto vertex_of = [&](uint64_t list) {
+        const uint32_t bit = (uint32_t)__builtin_ctzll(list | (1ull << 63));
+        const uint32_t jk = bit >> 4;
+        const uint32_t o = jk == 0 ? off[0] : (jk == 1 ? off[1] : (jk == 2 ? off[2] : off[3]));
+        return o + (bit & 15u);
+    };
+    float gn[3] = {0.0f, 0.0f, 0.0f}, gp[3] = {0.0f, 0.0f, 0.0f};
+    // (loaded unconditionally from a valid vertex and added as +0 when not
+    // on the list: a sum that starts at +0 never becomes -0, so adding +0
+    // changes no bits)
+    float2 pre[kPreload][3];
+    bool ok[kPreload];
+    {
+        uint64_t l = slots;
+#pragma unroll
+        for (int i = 0; i < kPreload; ++i) {
+            const uint32_t vi = vertex_of(l);
+            ok[i] = l != 0u && vi < nvert;
+            const float2* gi = (const float2*)(gout + (size_t)(ok[i] ? vi : 0u) * 6);
+            pre[i][0] = gi[0];
+            pre[i][1] = gi[1];
+            pre[i][2] = gi[2];
+            l &= l - 1u;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kPreload; ++i) {
+        gn[0] += ok[i] ? pre[i][0].x : 0.0f;
+        gn[1] += ok[i] ? pre[i][0].y : 0.0f;
+        gn[2] += ok[i] ? pre[i][1].x : 0.0f;
+        gp[0] += ok[i] ? pre[i][1].y : 0.0f;
+        gp[1] += ok[i] ? pre[i][2].x : 0.0f;
+        gp[2] += ok[i] ? pre[i][2].y : 0.0f;
+        slots &= slots - 1u;
+    }
+    while (slots) {
+        const uint32_t vi = vertex_of(slots);
+        slots &= slots - 1u;
+        if (vi >= nvert) continue;
+        const float2* gi = (const float2*)(gout + (size_t)vi * 6);
+        const float2 g01 = gi[0], g23 = gi[1], g45 = gi[2];
+        gn[0] += g01.x;
+        gn[1] += g01.y;
+        gn[2] += g23.x;
+        gp[0] += g23.y;
+        gp[1] += g45.x;
+        gp[2] += g45.y;
+    }
+    // the forward's t (hardware reciprocal, as emission computes it)
+    const float rD = __builtin_amdgcn_rcpf(va - vb);
+    const float tt = (va - T) * rD;
+    const float u = 1.0f - tt;
+    const float wx = gn[0] * g.scx, wy = gn[1] * g.scy, wz = gn[2] * g.scz;
+    float gt = axis == 0 ? gp[0] * g.scx : (axis == 1 ? gp[1] * g.scy : gp[2] * g.scz);
+    gt += wx * (nax - nbx) + wy * (nay - nby) + wz * (naz - nbz);
+    const float dva = gt * (T - vb) * rD * rD;
+    const float dvb = gt * (va - T) * rD * rD;
+    const float dnax = -wx * u, dnay = -wy * u, dnaz = -wz * u;
+    // contributions by kind: a, b (t and na_axis), a + e_p, a + e_q,
+    // b + e_x, b + e_y, b + e_z; then dL/dT
+    if (ri != kNoEdge) {
+        float* r = rec + ri;
+        r[0] = dva;
+        r[stride] = dvb + (axis == 0 ? dnax : (axis == 1 ? dnay : dnaz));
+        r[2 * stride] = axis == 0 ? dnay : dnax;
+        r[3 * stride] = axis == 2 ? dnay : dnaz;
+        r[4 * stride] = -wx * tt;
+        r[5 * stride] = -wy * tt;
+        r[6 * stride] = -wz * tt;
+    }
+    return (wx + wy + wz) - gt * rD;
+}
+
+// records pass.  Lanes take the quads of the cells pass to list the owned
+// edges (list: kEdgeListMax 16-bit entries of LDS per wave, Morton cell
+// index << 8 | edge << 4 | rank), then one edge each; a block with more
+// edges than the list holds has each lane take its own cells' edges.
+// smask: the slot masks of fill_slot_masks.  partials[item] = the item's
+// dL/dT sum (0 for further chunks).
+__device__ __forceinline__ void adjoint_records_body(const VolDesc* __restrict__ vols, float T,
+                                                     const uint2* __restrict__ items,
+                                                     const Totals* __restrict__ totals,
+                                                     const uint16_t (*smask)[12], const uint8_t* __restrict__ codes,
+                                                     const uint32_t* __restrict__ cell_off,
+                                                     const float* __restrict__ gout, uint32_t nvert,
+                                                     float* __restrict__ rec, size_t stride,
+                                                     float* __restrict__ partials, uint16_t* list)
+{
+    const uint32_t nitems = load_uniform(&totals->items);
+    const int lane = threadIdx.x & 63;
+    const uint32_t w0 = blockIdx.x * kAdjointWaves + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * kAdjointWaves;
+    const int qx = (int)morton_compact((uint32_t)lane) * 2, qy = (int)morton_compact((uint32_t)lane >> 1) * 2;
+    for (uint32_t it = w0; it < nitems; it += nw) {
+        const uint2 item = load_uniform(items + it);
+        if (item.y & 0xfu) {
+            if (lane == 0) partials[it] = 0.0f;
+            continue;
+        }
+        const VolDesc d = load_uniform(vols + item_volume(item.y));
+        const Geom& g = d.g;
+        int x0, y0, z;
+        block_origin(g, item.x, x0, y0, z);
+        const int side = 1 << g.blog2;
+        const uint64_t nsamp = (uint64_t)g.sx * g.sy * g.sz;
+        const __amdgpu_buffer_rsrc_t vcodes = buffer_of(codes + d.lattice_base, nsamp);
+        const __amdgpu_buffer_rsrc_t voffs = buffer_of(cell_off + d.lattice_base, nsamp * 4u);
+        uint32_t m[4], n = 0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int x = x0 + qx + (q & 1), y = y0 + qy + (q >> 1);
+            const bool valid = qx < side && qy < side && x < g.cx && y < g.cy;
+            m[q] = buffer_load_u8(vcodes, valid ? lattice_index(g, x, y, z) : kNoEdge);   // 0 when not valid
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            m[q] = code_edges(m[q]) & owned_edges(g, x0 + qx + (q & 1), y0 + qy + (q >> 1), z);
+            n += (uint32_t)__builtin_popcount(m[q]);
+        }
+        const uint32_t inc = wave_scan_u32<false>(n);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(inc, 63);
+        float dts = 0.0f;
+        if (total <= (uint32_t)kEdgeListMax) {
+            uint32_t pos = inc - n;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t mm = m[q], rank = 0u;
+                while (mm) {
+                    const uint32_t e = (uint32_t)__builtin_ctz(mm);
+                    mm &= mm - 1u;
+                    list[pos++] = (uint16_t)(((uint32_t)(4 * lane + q) << 8) | (e << 4) | rank++);
+                }
+            }
+            wave_lds_sync();
+            for (uint32_t i = (uint32_t)lane; i < total; i += 64) {
+                const uint32_t en = list[i];
+                const uint32_t c = en >> 8;
+                const int x = x0 + (int)morton_compact(c), y = y0 + (int)morton_compact(c >> 1);
+                const uint32_t ri = buffer_load_u32(voffs, lattice_index(g, x, y, z) * 4u) + (en & 0xfu);
+                dts += adjoint_edge_record(d, T, smask, vcodes, voffs, gout, nvert, x, y, z, (int)((en >> 4) & 0xfu),
+                                           rec, stride, ri < nvert ? ri : kNoEdge);
+            }
+        } else {
+#pragma unroll 1
+            for (int q = 0; q < 4; ++q) {
+                const int x = x0 + qx + (q & 1), y = y0 + qy + (q >> 1);
+                uint32_t mm = m[q];
+                uint32_t ri = mm ? buffer_load_u32(voffs, lattice_index(g, x, y, z) * 4u) : 0u;
+                while (mm) {
+                    const int e = __builtin_ctz(mm);
+                    mm &= mm - 1u;
+                    dts += adjoint_edge_record(d, T, smask, vcodes, voffs, gout, nvert, x, y, z, e, rec, stride,
+                                               ri < nvert ? ri : kNoEdge);
+                    ++ri;
+                }
+            }
+        }
+        dts = wave_sum_f(dts);
+        if (lane == 0) partials[it] = dts;
+        wave_lds_sync();   // the list is rewritten by the next item
+    }
+}
+
+// The flagged gather tiles in ascending order (list[0 .. *count)), one
+// 1024-thread workgroup; flags is 16-byte aligned and zero past ntiles up to
+// a multiple of 16.
+__device__ __forceinline__ void adjoint_tile_list_body(const uint8_t* __restrict__ flags, uint32_t ntiles,
+                                                       uint32_t* __restrict__ list, uint32_t* __restrict__ count)
+{
+    __shared__ uint32_t wsum[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    uint32_t done = 0u;
+    for (uint32_t base = 0; base < ntiles; base += 16u * blockDim.x) {
+        const uint32_t i0 = base + 16u * threadIdx.x;
+        const uint4 w = i0 < ntiles ? *(const uint4*)(flags + i0) : make_uint4(0u, 0u, 0u, 0u);
+        uint32_t bits = 0u;   // bit k: tile i0 + k flagged
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t word = k < 4 ? w.x : (k < 8 ? w.y : (k < 12 ? w.z : w.w));
+            bits |= ((word >> (8 * (k & 3))) & 0xffu) ? (1u << k) : 0u;
+        }
+        const uint32_t n = (uint32_t)__builtin_popcount(bits);
+        const uint32_t inc = wave_scan_u32<false>(n);
+        if (lane == 63) wsum[wv] = inc;
+        __syncthreads();
+        uint32_t pos = done + inc - n, round = 0u;
+        for (int k = 0; k < nwv; ++k) {
+            if (k < wv) pos += wsum[k];
+            round += wsum[k];
+        }
+        __syncthreads();
+        while (bits) {
+            list[pos++] = i0 + (uint32_t)__builtin_ctz(bits);
+            bits &= bits - 1u;
+        }
+        done += round;
+    }
+    if (threadIdx.x == 0) *count = done;
+}
+
+// gather pass: a persistent grid of kGatherThreads-thread workgroups over
+// the listed tiles (the rest of dL/dfield is zero-filled beforehand);
+// tile_vol[t] and tile0[v]: the volume of tile t and the first tile of
+// volume v.  tab is LDS.
+__device__ __forceinline__ void adjoint_gather_body(const VolDesc* __restrict__ vols,
+                                                    const uint32_t* __restrict__ tile_vol,
+                                                    const uint32_t* __restrict__ tile0,
+                                                    const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ count,
+                                                    const uint8_t* __restrict__ codes,
+                                                    const uint32_t* __restrict__ cell_off,
+                                                    const float* __restrict__ rec, size_t stride,
+                                                    uint32_t nvert, float* __restrict__ grad,
+                                                    uint32_t (*tab)[kGatherOrigins])
+{
+    const uint32_t ntl = load_uniform(count);
+    // the seven record planes (a record index is below 2^30)
+    __amdgpu_buffer_rsrc_t plane[kAdjointRecord];
+#pragma unroll
+    for (int k = 0; k < kAdjointRecord; ++k) plane[k] = buffer_of(rec + k * stride, (uint64_t)stride * 4u);
+    for (uint32_t li = blockIdx.x; li < ntl; li += gridDim.x) {
+        const uint32_t b = load_uniform(list + li);
+        const uint32_t v = load_uniform(tile_vol + b);
+        const VolDesc d = load_uniform(vols + v);
+        const Geom& g = d.g;
+        const uint32_t t = b - load_uniform(tile0 + v);
+        const uint32_t ntx = (uint32_t)(g.sx + kGatherX - 1) / kGatherX;
+        const uint32_t nty = (uint32_t)(g.sy + kGatherY - 1) / kGatherY;
+        const int lx = (int)(t % ntx) * kGatherX, ly = (int)((t / ntx) % nty) * kGatherY;
+        const int lz = (int)(t / (ntx * nty)) * kGatherZ;
+        const uint32_t lb = d.lattice_base;
+        const __amdgpu_buffer_rsrc_t offs = buffer_of(cell_off + lb, (uint64_t)g.sx * g.sy * g.sz * 4u);
+
+        // codes of the cells lo - 3 .. lo + tile - 1 (0 outside the grid)
+        uint32_t cv[kGatherCellLoads];
+#pragma unroll
+        for (int k = 0; k < kGatherCellLoads; ++k) {
+            const int i = threadIdx.x + kGatherThreads * k;
+            const int ux = i % kGatherCX, r = i / kGatherCX, uy = r % kGatherCY, uz = r / kGatherCY;
+            const int cx = lx - 3 + ux, cy = ly - 3 + uy, cz = lz - 3 + uz;
+            const bool in = i < kGatherCells && cx >= 0 && cy >= 0 && cz >= 0 && cx < g.cx && cy < g.cy && cz < g.cz;
+            cv[k] = in ? (uint32_t)codes[lb + lattice_index(g, cx, cy, cz)] : 0u;
+        }
+        // per edge origin lo - 2 .. lo + tile - 1 and axis: the edge's record
+        // (kNoEdge where none), written by the owners: every staged cell with
+        // a sign change enters its owned edges that start in the table
+#pragma unroll
+        for (int k = 0; k < kGatherOriginLoads; ++k) {
+            const int i = threadIdx.x + kGatherThreads * k;
+            if (i < kGatherOrigins) {
+                tab[0][i] = kNoEdge;
+                tab[1][i] = kNoEdge;
+                tab[2][i] = kNoEdge;
+            }
+        }
+        uint32_t om[kGatherCellLoads], oo[kGatherCellLoads];
+#pragma unroll
+        for (int k = 0; k < kGatherCellLoads; ++k) {
+            const int i = threadIdx.x + kGatherThreads * k;
+            const int ux = i % kGatherCX, r = i / kGatherCX, uy = r % kGatherCY, uz = r / kGatherCY;
+            const int cx = lx - 3 + ux, cy = ly - 3 + uy, cz = lz - 3 + uz;
+            om[k] = i < kGatherCells ? code_edges(cv[k]) & owned_edges(g, cx, cy, cz) : 0u;
+            // (cells without owned edges read past the buffer's end: unused)
+            oo[k] = buffer_load_u32(offs, om[k] ? lattice_index(g, cx, cy, cz) * 4u : kNoEdge);
+        }
+        __syncthreads();   // the table is cleared
+#pragma unroll
+        for (int k = 0; k < kGatherCellLoads; ++k) {
+            uint32_t m = om[k];
+            if (!m) continue;
+            const int i = threadIdx.x + kGatherThreads * k;
+            const int ux = i % kGatherCX, r = i / kGatherCX, uy = r % kGatherCY, uz = r / kGatherCY;
+            uint32_t ri = oo[k];
+            while (m) {
+                const int e = __builtin_ctz(m);
+                m &= m - 1u;
+                const int axis = e >> 2, s0 = e & 1, s1 = (e >> 1) & 1;
+                // origin in table coordinates (cell + edge offset + 1)
+                const int ox = ux - 1 + (axis == 0 ? 0 : s0);
+                const int oy = uy - 1 + (axis == 0 ? s0 : (axis == 1 ? 0 : s1));
+                const int oz = uz - 1 + (axis == 2 ? 0 : s1);
+                if (ox >= 0 && oy >= 0 && oz >= 0 && ox < kGatherOX && oy < kGatherOY && oz < kGatherOZ)
+                    tab[axis][(oz * kGatherOY + oy) * kGatherOX + ox] = ri < nvert ? ri : kNoEdge;
+                ++ri;
+            }
+        }
+        __syncthreads();
+        // per sample: the 21 (axis, kind) contributions in order, their
+        // records looked up first and loaded together; then, on the
+        // lattice's upper faces, those that reach it through the clamp
+#pragma unroll 1
+        for (int j = 0; j < 4; ++j) {
+            const int i = threadIdx.x + kGatherThreads * j;
+            const int sx = lx + (i & (kGatherX - 1)), sy = ly + ((i / kGatherX) & (kGatherY - 1));
+            const int sz = lz + i / (kGatherX * kGatherY);
+            if (sx >= g.sx || sy >= g.sy || sz >= g.sz) continue;
+            // record of the edge from s - (ox, oy, oz) along axis
+            auto at = [&](int axis, int ox, int oy, int oz) {
+                return tab[axis][((sz - oz - lz + 2) * kGatherOY + (sy - oy - ly + 2)) * kGatherOX +
+                                 (sx - ox - lx + 2)];
+            };
+            uint32_t en[21];
+            float val[21];
+#pragma unroll
+            for (int axis = 0; axis < 3; ++axis) {
+                const int ex = axis == 0, ey = axis == 1, ez = axis == 2;   // e_axis
+                const int px = axis != 0, py = axis == 0;                   // e_p (p < q the other axes)
+                const int qy = axis == 2, qz = axis != 2;                   // e_q
+                uint32_t* e7 = en + 7 * axis;
+                e7[0] = at(axis, 0, 0, 0);             // a
+                e7[1] = at(axis, ex, ey, ez);          // b
+                e7[2] = at(axis, px, py, 0);           // a + e_p
+                e7[3] = at(axis, 0, qy, qz);           // a + e_q
+                e7[4] = at(axis, ex + 1, ey, ez);      // b + e_x
+                e7[5] = at(axis, ex, ey + 1, ez);      // b + e_y
+                e7[6] = at(axis, ex, ey, ez + 1);      // b + e_z
+            }
+            // (kNoEdge reads past the plane's end: 0, and adding +0 to a
+            // sum that starts at +0 changes no bits)
+#pragma unroll
+            for (int c = 0; c < 21; ++c) val[c] = __builtin_bit_cast(float, buffer_load_u32(plane[c % 7], en[c] * 4u));
+            float acc = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 21; ++c) acc += val[c];
+            const bool topx = sx == g.sx - 1, topy = sy == g.sy - 1, topz = sz == g.sz - 1;
+            if (topx || topy || topz) {
+#pragma unroll
+                for (int axis = 0; axis < 3; ++axis) {
+                    const int ex = axis == 0, ey = axis == 1, ez = axis == 2;
+                    const bool topp = axis == 0 ? topy : topx, topq = axis == 2 ? topy : topz;
+                    uint32_t c7[5];
+                    c7[0] = topp ? at(axis, 0, 0, 0) : kNoEdge;       // a + e_p clamped to a
+                    c7[1] = topq ? at(axis, 0, 0, 0) : kNoEdge;       // a + e_q clamped to a
+                    c7[2] = topx ? at(axis, ex, ey, ez) : kNoEdge;    // b + e_x clamped to b
+                    c7[3] = topy ? at(axis, ex, ey, ez) : kNoEdge;
+                    c7[4] = topz ? at(axis, ex, ey, ez) : kNoEdge;
+                    float cv7[5];
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) cv7[c] = __builtin_bit_cast(float, buffer_load_u32(plane[2 + c], c7[c] * 4u));
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) acc += cv7[c];
+                }
+            }
+            grad[lb + lattice_index(g, sx, sy, sz)] = acc;
+        }
+        __syncthreads();   // tab is rewritten by the next tile
+    }
+}
+
+// dL/dT: the per-item sums in item order, one 1024-thread workgroup (a fixed
+// shape, so the same bits every launch).
+__device__ __forceinline__ void adjoint_sum_items_body(const float* __restrict__ partials,
+                                                       const Totals* __restrict__ totals, float* __restrict__ out)
+{
+    __shared__ float sh[16];
+    const uint32_t n = totals->items;
+    // thread t adds items t, t + 1024, ... in order, eight loads in flight
+    float s = 0.0f;
+    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += 8u * blockDim.x) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t i = i0 + (uint32_t)k * blockDim.x;
+            v[k] = i < n ? partials[i] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (i0 + (uint32_t)k * blockDim.x < n) s += v[k];
+    }
+    s = wave_sum_f(s);
     const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    if ((threadIdx.x & 63) == 0) sh[w] = s;
     __syncthreads();
-    float r = 0.0f;
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
-    __syncthreads();
-    return r;
-}
-
-__device__ __forceinline__ void sum_partials_body(const float* __restrict__ x, uint32_t n, float* __restrict__ part)
-{
-    __shared__ float sh[16];
-    const uint32_t per = (n + kReduceBlocks - 1) / kReduceBlocks;
-    const uint32_t lo = blockIdx.x * per, hi = min(lo + per, n);
-    float s = 0.0f;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) s += x[i];
-    s = block_sum_f(s, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-__device__ __forceinline__ void sum_final_body(const float* __restrict__ part, float* __restrict__ out)
-{
-    __shared__ float sh[16];
-    float s = 0.0f;
-    for (int i = threadIdx.x; i < kReduceBlocks; i += blockDim.x) s += part[i];
-    s = block_sum_f(s, sh);
-    if (threadIdx.x == 0) *out = s;
+    if (threadIdx.x == 0) {
+        float r = 0.0f;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+        *out = r;
+    }
 }
 
 // The level-4 counts in Morton key order (parity and debugging).
@@ -1691,7 +2072,7 @@ __device__ __forceinline__ void for_each_vertex(const TraversalView& v, const MK
     if (blockDim.x != 64 * kEmitWaves) return;   // the emission loop needs 4 waves (one table entry a thread)
     const uint32_t nitems = load_uniform(&v.totals->items);
     emit_body<MODE, false>(v.vols, mk, v.threshold, v.items, v.l4_off, nitems, nullptr, nullptr, 0xffffffffu,
-                           v.bnorm, sh, v.cases[threadIdx.x], AdjointIO{}, cb);
+                           v.bnorm, sh, v.cases[threadIdx.x], cb);
 }
 
 }  // namespace hpmc
@@ -1751,20 +2132,4 @@ __device__ __forceinline__ void for_each_vertex(const TraversalView& v, const MK
         auto mk = [](const hpmc::VolDesc& dd) { return MAKE_FIELD; };                         \
         hpmc::emit_body<MODE, BATCH>(vols, mk, T, items, l4_off, nitems, out, out_ce,         \
                                      capacity, bnorm, sh, case_word);                         \
-    }
-
-// Adjoint of continuous emission (kEmitAdjoint) over the same items.
-#define HPMC_DEFINE_ADJOINT_KERNEL(NAME, MAKE_FIELD, BATCH)                                    \
-    extern "C" __global__ __launch_bounds__(256) void NAME(                                   \
-        const hpmc::VolDesc* vols, float T, const uint2* items, const uint32_t* l4_off,        \
-        const hpmc::Totals* totals, uint32_t count, const uint64_t* cases, const float* gout,  \
-        uint32_t* keys, float* vals, float* dT)                                               \
-    {                                                                                         \
-        __shared__ hpmc::EmitShared sh;                                                       \
-        const uint32_t nitems = hpmc::load_uniform(&totals->items);                           \
-        const uint64_t case_word = cases[threadIdx.x];   /* stored to LDS by emit_body */     \
-        auto mk = [](const hpmc::VolDesc& dd) { return MAKE_FIELD; };                         \
-        hpmc::emit_body<hpmc::kEmitAdjoint, BATCH>(vols, mk, T, items, l4_off, nitems, nullptr, \
-                                                   nullptr, count, nullptr, sh, case_word,    \
-                                                   hpmc::AdjointIO{gout, keys, vals, dT});    \
     }

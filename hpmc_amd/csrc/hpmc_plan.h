@@ -179,6 +179,9 @@ struct Plan {
     hpmc::Totals* h_totals = nullptr;     // pinned
     hpmc::VolCount* d_vcounts = nullptr;
     hpmc::VolCount* h_vcounts = nullptr;  // pinned
+    uint32_t* d_gather_tile0 = nullptr;   // first adjoint gather tile of each volume
+    uint32_t* d_gather_vol = nullptr;     // volume of each adjoint gather tile
+    uint64_t n_gather_tiles = 0;
 };
 
 inline void plan_free(Plan& p)
@@ -195,6 +198,8 @@ inline void plan_free(Plan& p)
     if (p.h_totals) (void)hipHostFree(p.h_totals);
     if (p.d_vcounts) (void)hipFree(p.d_vcounts);
     if (p.h_vcounts) (void)hipHostFree(p.h_vcounts);
+    if (p.d_gather_tile0) (void)hipFree(p.d_gather_tile0);
+    if (p.d_gather_vol) (void)hipFree(p.d_gather_vol);
     p = Plan();
 }
 
@@ -368,6 +373,22 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
     PLAN_TRY(hipMemset(p.d_totals, 0, sizeof(hpmc::Totals)));
     PLAN_TRY(hipHostMalloc(&p.h_totals, sizeof(hpmc::Totals), hipHostMallocDefault));
     memset(p.h_totals, 0, sizeof(hpmc::Totals));
+    // adjoint gather tiles of each volume
+    std::vector<uint32_t> tile0(specs.size()), gather_vol;
+    for (size_t i = 0; i < specs.size(); ++i) {
+        const hpmc::Geom& g = p.vols[i].g;
+        tile0[i] = (uint32_t)gather_vol.size();
+        const uint64_t n = (uint64_t)((g.sx + hpmc::kGatherX - 1) / hpmc::kGatherX) *
+                           ((g.sy + hpmc::kGatherY - 1) / hpmc::kGatherY) *
+                           ((g.sz + hpmc::kGatherZ - 1) / hpmc::kGatherZ);
+        gather_vol.insert(gather_vol.end(), n, (uint32_t)i);
+    }
+    p.n_gather_tiles = gather_vol.size();
+    PLAN_TRY(hipMalloc(&p.d_gather_tile0, tile0.size() * sizeof(uint32_t)));
+    PLAN_TRY(hipMemcpy(p.d_gather_tile0, tile0.data(), tile0.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    PLAN_TRY(hipMalloc(&p.d_gather_vol, gather_vol.size() * sizeof(uint32_t)));
+    PLAN_TRY(hipMemcpy(p.d_gather_vol, gather_vol.data(), gather_vol.size() * sizeof(uint32_t),
+                       hipMemcpyHostToDevice));
     PLAN_TRY(hipMalloc(&p.d_vcounts, p.vols.size() * sizeof(hpmc::VolCount)));
     PLAN_TRY(hipHostMalloc(&p.h_vcounts, p.vols.size() * sizeof(hpmc::VolCount), hipHostMallocDefault));
     memset(p.h_vcounts, 0, p.vols.size() * sizeof(hpmc::VolCount));
