@@ -41,7 +41,7 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-# 48^3 = 110,592 samples: 17 key bits, sorted with 9-bit digits (two passes)
+# cubic and non-cubic lattices, anisotropic extents, a grid of 7 cells along x
 @pytest.mark.parametrize("shape,ext", [((40, 40, 40), (1.0, 1.0, 1.0)), ((37, 52, 29), (2.0, 0.5, 1.0)),
                                        ((48, 48, 48), (1.0, 1.0, 1.0)), ((8, 60, 60), (1.0, 1.0, 1.0))])
 def test_adjoint_matches_fp64_oracle(shape, ext):
@@ -64,6 +64,36 @@ def test_adjoint_matches_fp64_oracle(shape, ext):
     assert abs(float(gt) - ref_t) <= 1e-4 * max(abs(ref_t), 1.0)
     # samples no vertex reads stay exactly zero
     assert np.array_equal(got == 0, ref_f == 0)
+    iso.close()
+
+
+def test_adjoint_every_edge_changing_sign():
+    # a perturbed checkerboard: every lattice edge changes sign, so each cell
+    # of the top slice owns five edges and a 16x16 block of it more than
+    # the 1024 its edge list holds (each lane then takes its own cells'
+    # edges), and every gather tile's edge table is full
+    shape = (40, 36, 12)
+    x = np.arange(shape[0])[None, None, :]
+    y = np.arange(shape[1])[None, :, None]
+    z = np.arange(shape[2])[:, None, None]
+    rng = np.random.default_rng(13)
+    f = (((x + y + z) & 1) + 0.2 * rng.random((shape[2], shape[1], shape[0])) - 0.1).astype(np.float32)
+    T = 0.5
+    iso = IsoSurface(shape)
+    iso.set_volume(torch.from_numpy(f).cuda())
+    verts, _ = iso.extract_all(T)
+    n = verts.shape[0]
+    codes, cell, slot = _oracle_list(f, T)
+    assert len(cell) == n and n > 12 * 39 * 35 * 11 // 2
+    gout = torch.from_numpy(np.random.default_rng(4).standard_normal((n, 6)).astype(np.float32)).cuda()
+    gf, gt = iso.adjoint(gout)
+    gf2, gt2 = iso.adjoint(gout)
+    torch.cuda.synchronize()
+    assert torch.equal(gf, gf2) and torch.equal(gt, gt2)
+    ref_f, ref_t = oracle.adjoint(f, codes, cell, slot, T, gout.cpu().numpy().astype(np.float64))
+    got = gf.cpu().numpy().astype(np.float64)
+    assert _rel(got, ref_f) < 1e-4, _rel(got, ref_f)
+    assert abs(float(gt) - ref_t) <= 1e-4 * max(abs(ref_t), 1.0)
     iso.close()
 
 
