@@ -177,6 +177,18 @@ inline void adjoint_free(AdjointScratch& s)
         }                                                                                  \
     } while (0)
 
+// Workgroups of `threads` threads of kernel k resident on the current
+// device at once (occupancy x compute units), at least 1.
+inline unsigned resident_grid(const void* k, int threads)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, threads, 0) != hipSuccess)
+        return 1024;
+    return (unsigned)std::max(cus * per, 1);
+}
+
 // dL/dfield (n_lattice floats, every sample written) and dL/dT (one float)
 // of the last extraction of plan p (nvert vertices at threshold T), from
 // dL/dvertices gout (nvert x 6, normal then position).  Deterministic, with
@@ -244,7 +256,10 @@ inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, float T, uint32_t n
     uint32_t* count = s.tiles + p.n_gather_tiles;
     hipLaunchKernelGGL(hpmc_adjoint_tile_list, dim3(1), dim3(1024), 0, stream, (const uint8_t*)flags,
                        (uint32_t)p.n_gather_tiles, s.tiles, count);
-    hipLaunchKernelGGL(hpmc_adjoint_gather, dim3((unsigned)std::min<uint64_t>(p.n_gather_tiles, 2048)),
+    hipLaunchKernelGGL(hpmc_adjoint_gather,
+                       dim3((unsigned)std::min<uint64_t>(p.n_gather_tiles,
+                                                         resident_grid((const void*)hpmc_adjoint_gather,
+                                                                       hpmc::kGatherThreads))),
                        dim3(hpmc::kGatherThreads), 0, stream, (const hpmc::VolDesc*)p.d_vols,
                        (const uint32_t*)p.d_gather_vol, (const uint32_t*)p.d_gather_tile0, (const uint32_t*)s.tiles,
                        (const uint32_t*)count, (const uint8_t*)s.codes, (const uint32_t*)s.cell_off,

@@ -1871,19 +1871,10 @@ __device__ __forceinline__ void adjoint_gather_body(const VolDesc* __restrict__ 
         const uint32_t lb = d.lattice_base;
         const __amdgpu_buffer_rsrc_t offs = buffer_of(cell_off + lb, (uint64_t)g.sx * g.sy * g.sz * 4u);
 
-        // codes of the cells lo - 3 .. lo + tile - 1 (0 outside the grid)
-        uint32_t cv[kGatherCellLoads];
-#pragma unroll
-        for (int k = 0; k < kGatherCellLoads; ++k) {
-            const int i = threadIdx.x + kGatherThreads * k;
-            const int ux = i % kGatherCX, r = i / kGatherCX, uy = r % kGatherCY, uz = r / kGatherCY;
-            const int cx = lx - 3 + ux, cy = ly - 3 + uy, cz = lz - 3 + uz;
-            const bool in = i < kGatherCells && cx >= 0 && cy >= 0 && cz >= 0 && cx < g.cx && cy < g.cy && cz < g.cz;
-            cv[k] = in ? (uint32_t)codes[lb + lattice_index(g, cx, cy, cz)] : 0u;
-        }
         // per edge origin lo - 2 .. lo + tile - 1 and axis: the edge's record
-        // (kNoEdge where none), written by the owners: every staged cell with
-        // a sign change enters its owned edges that start in the table
+        // (kNoEdge where none), written by the owners among the cells
+        // lo - 3 .. lo + tile - 1.  A thread takes one column (x, y) of those
+        // cells and walks it along z.
 #pragma unroll
         for (int k = 0; k < kGatherOriginLoads; ++k) {
             const int i = threadIdx.x + kGatherThreads * k;
@@ -1893,35 +1884,71 @@ __device__ __forceinline__ void adjoint_gather_body(const VolDesc* __restrict__ 
                 tab[2][i] = kNoEdge;
             }
         }
-        uint32_t om[kGatherCellLoads], oo[kGatherCellLoads];
+        const int tid = threadIdx.x;
+        const bool colt = tid < kGatherCX * kGatherCY;
+        const int ux = tid % kGatherCX, uy = tid / kGatherCX;
+        const int cx = lx - 3 + ux, cy = ly - 3 + uy;
+        const bool colin = colt && cx >= 0 && cy >= 0 && cx < g.cx && cy < g.cy;
+        const uint32_t col = colin ? lattice_index(g, cx, cy, 0) : 0u;
+        const uint32_t zstride = (uint32_t)g.sx * (uint32_t)g.sy;
+        // the column's codes (0 outside the grid), then per cell the owned
+        // sign-changing edges (om, which ranks them) and the first vertex of
+        // the cells with edges to enter
+        uint32_t om[kGatherCZ], oo[kGatherCZ];
 #pragma unroll
-        for (int k = 0; k < kGatherCellLoads; ++k) {
-            const int i = threadIdx.x + kGatherThreads * k;
-            const int ux = i % kGatherCX, r = i / kGatherCX, uy = r % kGatherCY, uz = r / kGatherCY;
-            const int cx = lx - 3 + ux, cy = ly - 3 + uy, cz = lz - 3 + uz;
-            om[k] = i < kGatherCells ? code_edges(cv[k]) & owned_edges(g, cx, cy, cz) : 0u;
-            // (cells without owned edges read past the buffer's end: unused)
-            oo[k] = buffer_load_u32(offs, om[k] ? lattice_index(g, cx, cy, cz) * 4u : kNoEdge);
+        for (int uz = 0; uz < kGatherCZ; ++uz) {
+            const int cz = lz - 3 + uz;
+            om[uz] = (colin && cz >= 0 && cz < g.cz) ? (uint32_t)codes[lb + col + (uint32_t)cz * zstride] : 0u;
+        }
+        // An edge starts at the cell's origin + its offset, in table
+        // coordinates cell + offset - 1: outside the table for offset 0 on
+        // the region's first layer and for offset 1 on its last.
+        uint32_t keep = 0xfffu;
+        if (ux == 0) keep &= 0xaa0u;   // only x offset 1: y and z edges with s0 = 1
+        if (uy == 0) keep &= 0xc0au;   // only y offset 1: x edges with s0 = 1, z edges with s1 = 1
+        if (ux == kGatherCX - 1) keep &= ~0xaa0u;
+        if (uy == kGatherCY - 1) keep &= ~0xc0au;
+        const uint32_t own_xy = (cx < g.cx - 1 ? ~0xaa0u : 0xfffu) & (cy < g.cy - 1 ? ~0xc0au : 0xfffu);
+        auto keep_z = [&](int uz) {
+            return keep & (uz == 0 ? 0x0ccu : 0xfffu) & (uz == kGatherCZ - 1 ? ~0x0ccu : 0xfffu);
+        };
+#pragma unroll
+        for (int uz = 0; uz < kGatherCZ; ++uz) {
+            const int cz = lz - 3 + uz;
+            const uint32_t m = code_edges(om[uz]) & own_xy & (cz < g.cz - 1 ? ~0x0ccu : 0xfffu);
+            om[uz] = m;
+            // (cells without edges to enter read past the buffer's end: unused)
+            oo[uz] = buffer_load_u32(offs, (m & keep_z(uz)) ? (col + (uint32_t)cz * zstride) * 4u : kNoEdge);
         }
         __syncthreads();   // the table is cleared
 #pragma unroll
-        for (int k = 0; k < kGatherCellLoads; ++k) {
-            uint32_t m = om[k];
-            if (!m) continue;
-            const int i = threadIdx.x + kGatherThreads * k;
-            const int ux = i % kGatherCX, r = i / kGatherCX, uy = r % kGatherCY, uz = r / kGatherCY;
-            uint32_t ri = oo[k];
-            while (m) {
-                const int e = __builtin_ctz(m);
-                m &= m - 1u;
+        for (int uz = 0; uz < kGatherCZ; ++uz) {
+            const uint32_t m = om[uz];
+            uint32_t w = m & keep_z(uz);
+            if (!w) continue;
+            const uint32_t ri = oo[uz];
+            auto entry = [&](uint32_t e) {   // record of owned edge e: first vertex + rank
+                const uint32_t r = ri + (uint32_t)__builtin_popcount(m & ((1u << e) - 1u));
+                return r < nvert ? r : kNoEdge;
+            };
+            // the three edges from the cell's origin share one table slot
+            if (w & 0x111u) {
+                const int o0 = ((uz - 1) * kGatherOY + (uy - 1)) * kGatherOX + (ux - 1);
+                if (w & 0x001u) tab[0][o0] = entry(0);
+                if (w & 0x010u) tab[1][o0] = entry(4);
+                if (w & 0x100u) tab[2][o0] = entry(8);
+                w &= ~0x111u;
+            }
+            // edges from the cell's upper faces (cells on the grid's upper
+            // faces only)
+            while (w) {
+                const int e = __builtin_ctz(w);
+                w &= w - 1u;
                 const int axis = e >> 2, s0 = e & 1, s1 = (e >> 1) & 1;
-                // origin in table coordinates (cell + edge offset + 1)
                 const int ox = ux - 1 + (axis == 0 ? 0 : s0);
                 const int oy = uy - 1 + (axis == 0 ? s0 : (axis == 1 ? 0 : s1));
                 const int oz = uz - 1 + (axis == 2 ? 0 : s1);
-                if (ox >= 0 && oy >= 0 && oz >= 0 && ox < kGatherOX && oy < kGatherOY && oz < kGatherOZ)
-                    tab[axis][(oz * kGatherOY + oy) * kGatherOX + ox] = ri < nvert ? ri : kNoEdge;
-                ++ri;
+                tab[axis][(oz * kGatherOY + oy) * kGatherOX + ox] = entry((uint32_t)e);
             }
         }
         __syncthreads();
