@@ -81,14 +81,13 @@ extern "C" __global__ __launch_bounds__(256) void hpmc_adjoint_records(const hpm
                                                                        const hpmc::Totals* totals,
                                                                        const uint64_t* cases, const uint8_t* codes,
                                                                        const uint32_t* cell_off, const float* gout,
-                                                                       uint32_t nvert, float* rec, uint64_t stride,
-                                                                       float* partials)
+                                                                       uint32_t nvert, float* rec, float* partials)
 {
     __shared__ uint16_t smask[256][12];
     __shared__ uint16_t lists[hpmc::kAdjointWaves][hpmc::kEdgeListMax];
     hpmc::fill_slot_masks(cases, smask);
     __syncthreads();
-    hpmc::adjoint_records_body(vols, T, items, totals, smask, codes, cell_off, gout, nvert, rec, stride, partials,
+    hpmc::adjoint_records_body(vols, T, items, totals, smask, codes, cell_off, gout, nvert, rec, partials,
                                lists[threadIdx.x >> 6]);
 }
 
@@ -103,10 +102,10 @@ extern "C" __global__ __launch_bounds__(256) void hpmc_adjoint_gather(const hpmc
                                                                       const uint32_t* tile0, const uint32_t* list,
                                                                       const uint32_t* count, const uint8_t* codes,
                                                                       const uint32_t* cell_off, const float* rec,
-                                                                      uint64_t stride, uint32_t nvert, float* grad)
+                                                                      uint32_t nvert, float* grad)
 {
     __shared__ uint32_t tab[3][hpmc::kGatherOrigins];
-    hpmc::adjoint_gather_body(vols, tile_vol, tile0, list, count, codes, cell_off, rec, stride, nvert, grad, tab);
+    hpmc::adjoint_gather_body(vols, tile_vol, tile0, list, count, codes, cell_off, rec, nvert, grad, tab);
 }
 
 extern "C" __global__ __launch_bounds__(1024) void hpmc_adjoint_sum_items(const float* partials,
@@ -200,10 +199,9 @@ inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, float T, uint32_t n
                            float* grad_field, float* grad_iso, const uint64_t* cases, hipStream_t stream,
                            std::string* err)
 {
-    // 32-bit byte offsets into the record planes and a volume's first-vertex
-    // array
-    if (nvert >= (1u << 30)) {
-        *err = "too many vertices for the adjoint (at most 2^30 - 1)";
+    // 32-bit byte offsets into the records and a volume's first-vertex array
+    if (nvert >= (1u << 27)) {
+        *err = "too many vertices for the adjoint (at most 2^27 - 1)";
         return HPMC_ERR_SIZE;
     }
     for (const hpmc::VolDesc& d : p.vols) {
@@ -221,7 +219,7 @@ inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, float T, uint32_t n
         adjoint_free(s);
         ADJ_TRY(hipMalloc(&s.codes, (cl + 15) / 16 * 16 + (ct + 15) / 16 * 16));
         ADJ_TRY(hipMalloc(&s.cell_off, cl * sizeof(uint32_t)));
-        ADJ_TRY(hipMalloc(&s.rec, cv * hpmc::kAdjointRecord * sizeof(float)));
+        ADJ_TRY(hipMalloc(&s.rec, cv * hpmc::kAdjointRecordFloats * sizeof(float)));
         ADJ_TRY(hipMalloc(&s.partials, ci * sizeof(float)));
         ADJ_TRY(hipMalloc(&s.tiles, (ct + 1) * sizeof(uint32_t)));
         s.cap_vertices = cv;
@@ -249,8 +247,7 @@ inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, float T, uint32_t n
                        flags);
     hipLaunchKernelGGL(hpmc_adjoint_records, dim3(item_grid), dim3(64 * hpmc::kAdjointWaves), 0, stream,
                        (const hpmc::VolDesc*)p.d_vols, T, (const uint2*)p.items, (const hpmc::Totals*)p.d_totals,
-                       cases, (const uint8_t*)s.codes, (const uint32_t*)s.cell_off, gout, nvert, s.rec,
-                       (uint64_t)s.cap_vertices, s.partials);
+                       cases, (const uint8_t*)s.codes, (const uint32_t*)s.cell_off, gout, nvert, s.rec, s.partials);
     // dL/dfield: zero, then the flagged tiles gathered
     hipLaunchKernelGGL(hpmc_fill_f32, dim3(4096), dim3(256), 0, stream, grad_field, (uint64_t)p.n_lattice, 0.0f);
     uint32_t* count = s.tiles + p.n_gather_tiles;
@@ -263,7 +260,7 @@ inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, float T, uint32_t n
                        dim3(hpmc::kGatherThreads), 0, stream, (const hpmc::VolDesc*)p.d_vols,
                        (const uint32_t*)p.d_gather_vol, (const uint32_t*)p.d_gather_tile0, (const uint32_t*)s.tiles,
                        (const uint32_t*)count, (const uint8_t*)s.codes, (const uint32_t*)s.cell_off,
-                       (const float*)s.rec, (uint64_t)s.cap_vertices, nvert, grad_field);
+                       (const float*)s.rec, nvert, grad_field);
     hipLaunchKernelGGL(hpmc_adjoint_sum_items, dim3(1), dim3(1024), 0, stream, (const float*)s.partials,
                        (const hpmc::Totals*)p.d_totals, grad_iso);
     ADJ_TRY(hipGetLastError());

@@ -1417,7 +1417,8 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
 // the other axes); its owner is the first of those inside the grid in the
 // order (0,0), (1,0), (0,1), (1,1): a cell owns the three edges from its
 // origin and, on the grid's upper faces, the edges there.
-constexpr int kAdjointRecord = 7;           // contributions per edge record, one plane each
+constexpr int kAdjointRecord = 7;           // contributions per edge record
+constexpr int kAdjointRecordFloats = 8;     // floats a record takes (32 bytes)
 constexpr uint32_t kNoEdge = 0xffffffffu;
 constexpr int kAdjointWaves = 4;            // waves per workgroup of the cells and records passes
 constexpr int kEdgeListMax = 1024;          // listed edges of a block (more: each lane takes its own)
@@ -1584,9 +1585,9 @@ __device__ __forceinline__ void adjoint_cells_body(const VolDesc* __restrict__ v
 }
 
 // The record of edge e of cell (x, y, z) (an owned sign-changing edge);
-// returns its dL/dT term and stores contribution k at rec[k stride + ri]
-// unless ri is kNoEdge (records are stored as seven planes of stride floats,
-// so that a gather of one kind reads with 32-bit element offsets).  vcodes
+// returns its dL/dT term and stores the record (contribution k at float k
+// of 32 bytes) at rec + 32 ri unless ri is kNoEdge: one sector per edge,
+// which the gather's up to seven reads of the edge share.  vcodes
 // and voffs: buffers over the volume's codes and first vertices; smask: the
 // slot masks of fill_slot_masks.
 // cases: the 256 case words (LDS).  Vertices at or past nvert (only if the
@@ -1595,8 +1596,7 @@ __device__ __forceinline__ void adjoint_cells_body(const VolDesc* __restrict__ v
 __device__ __forceinline__ float adjoint_edge_record(const VolDesc& d, float T, const uint16_t (*smask)[12],
                                                      __amdgpu_buffer_rsrc_t vcodes, __amdgpu_buffer_rsrc_t voffs,
                                                      const float* __restrict__ gout, uint32_t nvert, int x, int y,
-                                                     int z, int e, float* __restrict__ rec, size_t stride,
-                                                     uint32_t ri)
+                                                     int z, int e, float* __restrict__ rec, uint32_t ri)
 {
     const Geom& g = d.g;
     const int axis = e >> 2, s0 = e & 1, s1 = (e >> 1) & 1;
@@ -1699,14 +1699,10 @@ __device__ __forceinline__ float adjoint_edge_record(const VolDesc& d, float T, 
     // contributions by kind: a, b (t and na_axis), a + e_p, a + e_q,
     // b + e_x, b + e_y, b + e_z; then dL/dT
     if (ri != kNoEdge) {
-        float* r = rec + ri;
-        r[0] = dva;
-        r[stride] = dvb + (axis == 0 ? dnax : (axis == 1 ? dnay : dnaz));
-        r[2 * stride] = axis == 0 ? dnay : dnax;
-        r[3 * stride] = axis == 2 ? dnay : dnaz;
-        r[4 * stride] = -wx * tt;
-        r[5 * stride] = -wy * tt;
-        r[6 * stride] = -wz * tt;
+        float4* r = (float4*)(rec + (size_t)ri * kAdjointRecordFloats);
+        r[0] = make_float4(dva, dvb + (axis == 0 ? dnax : (axis == 1 ? dnay : dnaz)), axis == 0 ? dnay : dnax,
+                           axis == 2 ? dnay : dnaz);
+        r[1] = make_float4(-wx * tt, -wy * tt, -wz * tt, 0.0f);
     }
     return (wx + wy + wz) - gt * rD;
 }
@@ -1723,7 +1719,7 @@ __device__ __forceinline__ void adjoint_records_body(const VolDesc* __restrict__
                                                      const uint16_t (*smask)[12], const uint8_t* __restrict__ codes,
                                                      const uint32_t* __restrict__ cell_off,
                                                      const float* __restrict__ gout, uint32_t nvert,
-                                                     float* __restrict__ rec, size_t stride,
+                                                     float* __restrict__ rec,
                                                      float* __restrict__ partials, uint16_t* list)
 {
     const uint32_t nitems = load_uniform(&totals->items);
@@ -1778,7 +1774,7 @@ __device__ __forceinline__ void adjoint_records_body(const VolDesc* __restrict__
                 const int x = x0 + (int)morton_compact(c), y = y0 + (int)morton_compact(c >> 1);
                 const uint32_t ri = buffer_load_u32(voffs, lattice_index(g, x, y, z) * 4u) + (en & 0xfu);
                 dts += adjoint_edge_record(d, T, smask, vcodes, voffs, gout, nvert, x, y, z, (int)((en >> 4) & 0xfu),
-                                           rec, stride, ri < nvert ? ri : kNoEdge);
+                                           rec, ri < nvert ? ri : kNoEdge);
             }
         } else {
 #pragma unroll 1
@@ -1789,7 +1785,7 @@ __device__ __forceinline__ void adjoint_records_body(const VolDesc* __restrict__
                 while (mm) {
                     const int e = __builtin_ctz(mm);
                     mm &= mm - 1u;
-                    dts += adjoint_edge_record(d, T, smask, vcodes, voffs, gout, nvert, x, y, z, e, rec, stride,
+                    dts += adjoint_edge_record(d, T, smask, vcodes, voffs, gout, nvert, x, y, z, e, rec,
                                                ri < nvert ? ri : kNoEdge);
                     ++ri;
                 }
@@ -1849,15 +1845,14 @@ __device__ __forceinline__ void adjoint_gather_body(const VolDesc* __restrict__ 
                                                     const uint32_t* __restrict__ count,
                                                     const uint8_t* __restrict__ codes,
                                                     const uint32_t* __restrict__ cell_off,
-                                                    const float* __restrict__ rec, size_t stride,
+                                                    const float* __restrict__ rec,
                                                     uint32_t nvert, float* __restrict__ grad,
                                                     uint32_t (*tab)[kGatherOrigins])
 {
     const uint32_t ntl = load_uniform(count);
-    // the seven record planes (a record index is below 2^30)
-    __amdgpu_buffer_rsrc_t plane[kAdjointRecord];
-#pragma unroll
-    for (int k = 0; k < kAdjointRecord; ++k) plane[k] = buffer_of(rec + k * stride, (uint64_t)stride * 4u);
+    // the records (indices below 2^27: 32-bit byte offsets; kNoEdge reads
+    // past the end)
+    const __amdgpu_buffer_rsrc_t recs = buffer_of(rec, (uint64_t)nvert * 4u * kAdjointRecordFloats);
     for (uint32_t li = blockIdx.x; li < ntl; li += gridDim.x) {
         const uint32_t b = load_uniform(list + li);
         const uint32_t v = load_uniform(tile_vol + b);
@@ -1985,7 +1980,7 @@ __device__ __forceinline__ void adjoint_gather_body(const VolDesc* __restrict__ 
             // (kNoEdge reads past the plane's end: 0, and adding +0 to a
             // sum that starts at +0 changes no bits)
 #pragma unroll
-            for (int c = 0; c < 21; ++c) val[c] = __builtin_bit_cast(float, buffer_load_u32(plane[c % 7], en[c] * 4u));
+            for (int c = 0; c < 21; ++c) val[c] = __builtin_bit_cast(float, buffer_load_u32(recs, en[c] * (4u * kAdjointRecordFloats) + 4u * (c % 7)));
             float acc = 0.0f;
 #pragma unroll
             for (int c = 0; c < 21; ++c) acc += val[c];
@@ -2003,7 +1998,7 @@ __device__ __forceinline__ void adjoint_gather_body(const VolDesc* __restrict__ 
                     c7[4] = topz ? at(axis, ex, ey, ez) : kNoEdge;
                     float cv7[5];
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) cv7[c] = __builtin_bit_cast(float, buffer_load_u32(plane[2 + c], c7[c] * 4u));
+                    for (int c = 0; c < 5; ++c) cv7[c] = __builtin_bit_cast(float, buffer_load_u32(recs, c7[c] * (4u * kAdjointRecordFloats) + 4u * (2 + c)));
 #pragma unroll
                     for (int c = 0; c < 5; ++c) acc += cv7[c];
                 }
