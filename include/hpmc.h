@@ -261,8 +261,10 @@ int32_t HPMCcheckFieldSnippet(const char* source, uint32_t flags, char* log, siz
  * and dL/dT (device, one FP32), through the edge interpolation
  * t = (f(a)-T)/(f(a)-f(b)), the positions and the forward-difference
  * normals.  Interpolation weights are recomputed, not stored; the result is
- * deterministic (per-vertex contributions, stable sort by sample, in-order
- * sums).  Continuous FP32 volume fields only; waits for the count. */
+ * deterministic (the vertices of each lattice edge summed in a fixed order,
+ * each sample's edge contributions summed in a fixed order, no atomics).
+ * Continuous FP32 volume fields only, volumes of at most 2^30 samples and
+ * fewer than 2^27 vertices (HPMC_ERR_SIZE otherwise); waits for the count. */
 int32_t HPMCadjoint(struct HPMCTraversalHandle* th, const float* grad_vertices, float* grad_field, float* grad_iso);
 
 /* Layout of the reference base level (hpmc/src/init.cpp:105-130):
