@@ -67,6 +67,33 @@ def test_adjoint_matches_fp64_oracle(shape, ext):
     iso.close()
 
 
+@pytest.mark.parametrize("shape,grid", [((40, 36, 30), (30, 25, 20)), ((33, 20, 17), (32, 6, 9))])
+def test_adjoint_grid_smaller_than_lattice(shape, grid):
+    # cells of a grid smaller than the lattice: edges on the grid's upper
+    # faces are owned by the last cells, and the normals' forward samples
+    # past the grid are real lattice samples (no clamp) until the lattice ends
+    f = sphere_noise(shape, 21)
+    T = 0.0
+    iso = IsoSurface(shape, grid=grid)
+    iso.set_volume(torch.from_numpy(f).cuda())
+    verts, _ = iso.extract_all(T)
+    n = verts.shape[0]
+    assert n > 100
+    codes, counts = oracle.classify(f, T, cells=grid, size=shape)
+    cell, slot = oracle.vertex_list(counts, grid, oracle.layout(shape, grid))
+    assert len(cell) == n
+    gout = torch.from_numpy(np.random.default_rng(8).standard_normal((n, 6)).astype(np.float32)).cuda()
+    gf, gt = iso.adjoint(gout)
+    torch.cuda.synchronize()
+    ref_f, ref_t = oracle.adjoint(f, codes, cell, slot, T, gout.cpu().numpy().astype(np.float64), cells=grid,
+                                  size=shape)
+    got = gf.cpu().numpy().astype(np.float64)
+    assert _rel(got, ref_f) < 1e-4, _rel(got, ref_f)
+    assert abs(float(gt) - ref_t) <= 1e-4 * max(abs(ref_t), 1.0)
+    assert np.array_equal(got == 0, ref_f == 0)
+    iso.close()
+
+
 def test_adjoint_every_edge_changing_sign():
     # a perturbed checkerboard: every lattice edge changes sign, so each cell
     # of the top slice owns five edges and a 16x16 block of it more than
