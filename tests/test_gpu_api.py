@@ -1,7 +1,8 @@
 """API behaviour on the GPU beyond parity: stream hand-off, a field volume
 reset to NULL, bounded profiling events, graph replays at their captured
-threshold, argument checks of the adjoint and the autograd wrappers'
-handling of in-place edits and repeated backward passes."""
+threshold, argument checks of the adjoint, the autograd wrappers'
+handling of in-place edits and repeated backward passes, and sizes beyond
+the engine's offsets failing with an error code and recovering."""
 import ctypes
 
 import numpy as np
@@ -162,3 +163,33 @@ def test_autograd_detects_in_place_edits_and_allows_retained_graphs():
     gb1 = torch.autograd.grad(loss_b, fields, retain_graph=True)
     gb2 = torch.autograd.grad(loss_b, fields)
     assert all(torch.equal(x, y) for x, y in zip(gb1, gb2))
+
+
+def test_absurd_sizes_fail_cleanly_and_recover():
+    # The largest lattice the reference accepts, 16384^3 (a snippet field, so
+    # nothing of that size is ever allocated), has 2^34 level-4 keys, more
+    # than this build's 32-bit offsets address: the build fails with an
+    # error code before any allocation or launch, latches the HistoPyramid
+    # broken and counts 0; a reconfiguration recovers it.
+    n = 16384
+    iso = IsoSurface((n, n, n))
+    iso.set_snippet(workloads.CAYLEY_SNIPPET)
+    L = hpmc_amd.lib()
+    L.HPMCbuildHistopyramid(iso.h, 0.0)
+    assert hpmc_amd.last_error(iso.h)[0] == 3          # HPMC_ERR_SIZE
+    assert L.HPMCacquireNumberOfVertices(iso.h) == 0
+    L.HPMCsetLatticeSize(iso.h, 48, 40, 36)
+    L.HPMCbuildHistopyramid(iso.h, 0.0)
+    assert hpmc_amd.last_error(iso.h)[0] == 0
+    assert L.HPMCacquireNumberOfVertices(iso.h) > 0
+    iso.close()
+    # a batch volume of that size: refused at the batch's build, before any
+    # launch (the pointer given is never read)
+    small = torch.zeros((8, 8, 8), device="cuda")
+    b = Batch()
+    idx = L.HPMCbatchAddVolume(b.b, ctypes.c_void_p(small.data_ptr()), n, n, n, 1.0, 1.0, 1.0)
+    if idx >= 0:
+        assert L.HPMCbatchBuild(b.b, 0.0) == 3         # HPMC_ERR_SIZE
+    else:
+        assert -idx == 3
+    b.close()
