@@ -210,6 +210,11 @@ inline int32_t adjoint_run(const Plan& p, AdjointScratch& s, float T, uint32_t n
             return HPMC_ERR_SIZE;
         }
     }
+    // 32-bit sample keys into the concatenated lattice of a batch
+    if (p.n_lattice > 0xffffffffull) {
+        *err = "batch too large for the adjoint (at most 2^32 - 1 samples in all)";
+        return HPMC_ERR_SIZE;
+    }
     const uint64_t need_v = std::max<uint64_t>(nvert, 1), need_l = std::max<uint64_t>(p.n_lattice, 1);
     const uint64_t need_i = std::max<uint64_t>(p.n_items_max, 1), need_t = std::max<uint64_t>(p.n_gather_tiles, 1);
     if (need_v > s.cap_vertices || need_l > s.cap_lattice || need_i > s.cap_items || need_t > s.cap_tiles) {
@@ -856,6 +861,10 @@ bool extract_helper(HPMCTraversalHandle* th)
         fail(h, HPMC_ERR_ARG, "no output buffer bound to the traversal handle");
         return false;
     }
+    if (th->out_ce && !hpmc_host::plan_cells_fit_u32(h->plan)) {
+        fail(h, HPMC_ERR_SIZE, "(cell, edge) records index cells in 32 bits; the grid has more cells");
+        return false;
+    }
     uint64_t cap = th->capacity;
     if (th->out_ce && th->ce_capacity < cap) cap = th->ce_capacity;
     if (cap > 0xffffffffull) cap = 0xffffffffull;
@@ -1482,6 +1491,8 @@ int32_t batch_enqueue_build(HPMCBatch* b, float T)
 
 int32_t batch_enqueue_extract(HPMCBatch* b, float T, float* out, uint64_t capacity, void* cell_edge)
 {
+    if (cell_edge && !hpmc_host::plan_cells_fit_u32(b->plan))
+        return fail(nullptr, HPMC_ERR_SIZE, "(cell, edge) records index cells in 32 bits; a volume has more cells");
     const uint32_t cap = (uint32_t)std::min<uint64_t>(capacity, 0xffffffffull);
     BATCH_TRY(hpmc_host::plan_emit(b->plan, hpmc_host::Kernel{(const void*)hpmc_emit_volume_batch, nullptr}, T, out,
                                    (hpmc::CellEdge*)cell_edge, cap, b->constants->d_cases, b->constants->d_bnorm,

@@ -243,7 +243,8 @@ constexpr int kStreamLoad = 2;
 struct VolumeField {
     const float* __restrict__ v;
     int sx, sy, sz;
-    // linear index of the (clamped) sample a fetch at (i, j, k) reads
+    // linear index of the (clamped) sample a fetch at (i, j, k) reads, in
+    // 32 bits: only for volumes of fewer than 2^32 samples (the adjoint's)
     __device__ __forceinline__ uint32_t index(int i, int j, int k) const
     {
         i = min(max(i, 0), sx - 1);
@@ -251,7 +252,13 @@ struct VolumeField {
         k = min(max(k, 0), sz - 1);
         return ((uint32_t)k * (uint32_t)sy + (uint32_t)j) * (uint32_t)sx + (uint32_t)i;
     }
-    __device__ __forceinline__ float at(int i, int j, int k) const { return v[index(i, j, k)]; }
+    __device__ __forceinline__ float at(int i, int j, int k) const
+    {
+        i = min(max(i, 0), sx - 1);
+        j = min(max(j, 0), sy - 1);
+        k = min(max(k, 0), sz - 1);
+        return v[((size_t)k * sy + j) * sx + i];
+    }
     // Plane-wise access goes through a buffer resource per plane (a scalar
     // 128-bit descriptor) and 32-bit per-lane byte offsets.
     typedef __amdgpu_buffer_rsrc_t Plane;
@@ -311,14 +318,13 @@ struct VolumeStreamField : VolumeField {
 struct Volume8Field {
     const uint8_t* __restrict__ v;
     int sx, sy, sz;
-    __device__ __forceinline__ uint32_t index(int i, int j, int k) const
+    __device__ __forceinline__ float at(int i, int j, int k) const
     {
         i = min(max(i, 0), sx - 1);
         j = min(max(j, 0), sy - 1);
         k = min(max(k, 0), sz - 1);
-        return ((uint32_t)k * (uint32_t)sy + (uint32_t)j) * (uint32_t)sx + (uint32_t)i;
+        return (float)v[((size_t)k * sy + j) * sx + i] / 255.0f;
     }
-    __device__ __forceinline__ float at(int i, int j, int k) const { return (float)v[index(i, j, k)] / 255.0f; }
     typedef __amdgpu_buffer_rsrc_t Plane;
     __device__ __forceinline__ uint32_t lane_index(int i, int j) const
     {
@@ -369,20 +375,24 @@ struct Volume4Field {
     {
         i = min(max(i, 0), sx - 1);
         j = min(max(j, 0), sy - 1);
-        return ((uint32_t)j * (uint32_t)sx + (uint32_t)i) * 16u + 12u;
+        return ((uint32_t)j * (uint32_t)sx + (uint32_t)i) * 16u;
     }
+    // A plane's resource starts at the alpha of its first texel, so texel
+    // offsets are multiples of 16 and the last alpha ends at 16 (sx sy) - 12
+    // bytes, which fits the 32-bit record count up to a 16384 x 16384 plane.
     __device__ __forceinline__ Plane plane(int k) const
     {
         k = min(max(k, 0), sz - 1);
-        const float4* base = v + (size_t)k * sy * sx;
-        return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, sx * sy * 16, 0x00020000);
+        const char* base = (const char*)(v + (size_t)k * sy * sx) + 12;
+        return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)(uint32_t)((uint64_t)sx * sy * 16u - 12u),
+                                                 0x00020000);
     }
     __device__ __forceinline__ float fetch(Plane p, uint32_t idx) const
     {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p, idx, 0, 0));
     }
     __device__ __forceinline__ float sample(Plane p, uint32_t idx) const { return fetch(p, idx); }
-    __device__ __forceinline__ uint32_t col_index(int i) const { return (uint32_t)min(max(i, 0), sx - 1) * 16u + 12u; }
+    __device__ __forceinline__ uint32_t col_index(int i) const { return (uint32_t)min(max(i, 0), sx - 1) * 16u; }
     __device__ __forceinline__ uint32_t row_index(int j) const
     {
         return (uint32_t)min(max(j, 0), sy - 1) * (uint32_t)sx * 16u;

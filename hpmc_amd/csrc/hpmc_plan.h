@@ -285,8 +285,21 @@ inline int32_t plan_describe(Plan& p, const VolumeSpec* specs, size_t n, Layout*
         d.g = make_geom(specs[i], lays[i], zseg);
         d.field = specs[i].field;
         const uint64_t side = (uint64_t)1 << d.g.blocks_log2;
-        d.n_level4 = (uint32_t)(side * side);
-        d.n_cc = (uint32_t)((uint64_t)d.g.nbx * d.g.nby * d.g.npz * hpmc::kColStride);
+        const uint64_t n_level4 = side * side;
+        const uint64_t n_cc = (uint64_t)d.g.nbx * d.g.nby * d.g.npz * hpmc::kColStride;
+        // a block of b x b cells holds at most 15 b^2 vertices: 15 work
+        // items per slice of a 16x16 block, fewer for narrower blocks
+        const uint64_t bcells = (uint64_t)1 << (2 * d.g.blog2);
+        const uint64_t n_items =
+            (uint64_t)d.g.nbx * d.g.nby * d.g.cz * ((15 * bcells + hpmc::kEmitChunk - 1) / hpmc::kEmitChunk);
+        const uint64_t wgs = (uint64_t)classify_bands(d.g, p.cls_waves) * d.g.npz;
+        if (p.n_l4 + n_level4 > 0xffffffffull || p.n_cc + n_cc > 0xffffffffull ||
+            p.n_items_max + n_items > 0xffffffffull || p.n_wg + wgs > 0xffffffffull) {
+            *err = "build too large for 32-bit offsets";
+            return HPMC_ERR_SIZE;
+        }
+        d.n_level4 = (uint32_t)n_level4;
+        d.n_cc = (uint32_t)n_cc;
         d.colcnt_off = p.n_cc;
         d.l4_base = (uint32_t)p.n_l4;
         d.cls_wg0 = p.n_wg;
@@ -294,26 +307,14 @@ inline int32_t plan_describe(Plan& p, const VolumeSpec* specs, size_t n, Layout*
         d.n_l4tiles = (d.n_level4 + hpmc::kScanTile - 1) / hpmc::kScanTile;
         d.col_tile0 = p.n_colt;
         d.n_coltiles = (d.n_cc + hpmc::kScanTile - 1) / hpmc::kScanTile;
-        d.lattice_base = (uint32_t)p.n_lattice;
-        const uint64_t wgs = (uint64_t)classify_bands(d.g, p.cls_waves) * d.g.npz;
-        if (p.n_l4 + d.n_level4 > 0xffffffffull || p.n_wg + wgs > 0xffffffffull) {
-            *err = "build too large for 32-bit offsets";
-            return HPMC_ERR_SIZE;
-        }
+        d.lattice_base = (uint32_t)p.n_lattice;   // the adjoint takes builds under 2^32 samples
         p.n_l4t += d.n_l4tiles;
         p.n_colt += d.n_coltiles;
         p.n_cc += d.n_cc;
         p.n_l4 += d.n_level4;
         p.n_wg += (uint32_t)wgs;
         p.n_lattice += (uint64_t)d.g.sx * d.g.sy * d.g.sz;
-        if (p.n_lattice >= 0xffffffffull) {
-            *err = "build too large for 32-bit sample keys";
-            return HPMC_ERR_SIZE;
-        }
-        // a block of b x b cells holds at most 15 b^2 vertices: 15 work
-        // items per slice of a 16x16 block, fewer for narrower blocks
-        const uint64_t bcells = (uint64_t)1 << (2 * d.g.blog2);
-        p.n_items_max += (uint64_t)d.g.nbx * d.g.nby * d.g.cz * ((15 * bcells + hpmc::kEmitChunk - 1) / hpmc::kEmitChunk);
+        p.n_items_max += n_items;
     }
 
     // persistent emit grid: as many 4-wave workgroups as fit at once, a
@@ -469,6 +470,15 @@ inline hipError_t plan_classify_small(const Plan& p, const Kernel& k, float T, c
     void* args[] = {&vols, &Ta, &cc, &cs, &ts, &nl};
     const unsigned grid = (unsigned)((p.n_cc + 255) / 256);
     return launch(k, grid, 256, args, stream);
+}
+
+// True when every volume's cell indices fit the 32-bit cell field of the
+// (cell, edge) records.
+inline bool plan_cells_fit_u32(const Plan& p)
+{
+    for (const hpmc::VolDesc& d : p.vols)
+        if ((uint64_t)d.g.cx * d.g.cy * d.g.cz > 0xffffffffull) return false;
+    return true;
 }
 
 // True when the plan's (single) volume has blocks narrower than 16 cells.
