@@ -137,6 +137,7 @@ extern "C" __global__ __launch_bounds__(256) void hpmc_noalloc_begin(hpmc::VolDe
         *dst = d;
         state->done = 0u;
     }
+    if (threadIdx.x < hpmc::kScanGroups) state->group_done[threadIdx.x] = 0u;
     for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) tile_sums[i] = make_uint2(0u, 0u);
 }
 

@@ -220,6 +220,31 @@ __device__ __forceinline__ T load_uniform(const T* p)
     return r;
 }
 
+// A wave-uniform value loaded through the vector memory path, into the same
+// vector register of every lane (make it uniform with v_readfirstlane at
+// its use).  For prefetches inside a loop that also waits on LDS: a scalar
+// load counts on lgkmcnt, and because scalar loads may complete out of
+// order, any LDS wait issued while one is in flight has to wait for it too
+// (lgkmcnt(0)), putting a memory round trip into every LDS wait; vector
+// loads count on vmcnt, which completes in order and is waited on for the
+// prefetched planes anyway.
+__device__ __forceinline__ uint32_t prefetch_u32(const uint32_t* p)
+{
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(
+        __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, 4, 0x00020000), 0, 0, 0);
+}
+__device__ __forceinline__ uint2 prefetch_u2(const uint2* p)
+{
+    typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+    const v2u v = __builtin_amdgcn_raw_buffer_load_b64(
+        __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, 8, 0x00020000), 0, 0, 0);
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ uint2 uniform_u2(uint2 v)
+{
+    return make_uint2(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y));
+}
+
 // The value of lane - 1 (0 in lane 0): DPP wave_shr:1.
 __device__ __forceinline__ uint32_t wave_shr1_u32(uint32_t x)
 {
@@ -291,6 +316,24 @@ struct VolumeField {
     }
     // the sample value emission interpolates (fetch returns it as is)
     __device__ __forceinline__ float sample(Plane p, uint32_t idx) const { return fetch(p, idx); }
+    // Windows that lie inside the plane, read without clamping: a per-lane
+    // offset of the window sample (computed once per volume) and the
+    // wave-uniform offset of the window's corner, the scalar offset of the
+    // load, so that a window's loads take no per-item address arithmetic.
+    static constexpr bool kWindows = true;
+    __device__ __forceinline__ bool window_inside(int x0, int y0, int w) const { return x0 + w <= sx && y0 + w <= sy; }
+    __device__ __forceinline__ uint32_t window_lane(int kx, int ky) const
+    {
+        return ((uint32_t)ky * (uint32_t)sx + (uint32_t)kx) * 4u;
+    }
+    __device__ __forceinline__ uint32_t window_base(int x0, int y0) const
+    {
+        return ((uint32_t)y0 * (uint32_t)sx + (uint32_t)x0) * 4u;
+    }
+    __device__ __forceinline__ float window_sample(Plane p, uint32_t lane_off, uint32_t base) const
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p, lane_off, base, 0));
+    }
 };
 
 // An FP32 volume whose classification row loads are non-temporal: for
@@ -352,6 +395,21 @@ struct Volume8Field {
         return (float)__builtin_amdgcn_raw_buffer_load_b8(p, col, row, 0);
     }
     __device__ __forceinline__ float sample(Plane p, uint32_t idx) const { return fetch(p, idx) / 255.0f; }
+    // windows inside the plane without clamping (as VolumeField's)
+    static constexpr bool kWindows = true;
+    __device__ __forceinline__ bool window_inside(int x0, int y0, int w) const { return x0 + w <= sx && y0 + w <= sy; }
+    __device__ __forceinline__ uint32_t window_lane(int kx, int ky) const
+    {
+        return (uint32_t)ky * (uint32_t)sx + (uint32_t)kx;
+    }
+    __device__ __forceinline__ uint32_t window_base(int x0, int y0) const
+    {
+        return (uint32_t)y0 * (uint32_t)sx + (uint32_t)x0;
+    }
+    __device__ __forceinline__ float window_sample(Plane p, uint32_t lane_off, uint32_t base) const
+    {
+        return (float)__builtin_amdgcn_raw_buffer_load_b8(p, lane_off, base, 0) / 255.0f;
+    }
 };
 
 // RGBA FP32 volume carrying the gradient in rgb and the scalar in alpha, the
@@ -392,6 +450,21 @@ struct Volume4Field {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p, idx, 0, 0));
     }
     __device__ __forceinline__ float sample(Plane p, uint32_t idx) const { return fetch(p, idx); }
+    // windows inside the plane without clamping (as VolumeField's)
+    static constexpr bool kWindows = true;
+    __device__ __forceinline__ bool window_inside(int x0, int y0, int w) const { return x0 + w <= sx && y0 + w <= sy; }
+    __device__ __forceinline__ uint32_t window_lane(int kx, int ky) const
+    {
+        return ((uint32_t)ky * (uint32_t)sx + (uint32_t)kx) * 16u;
+    }
+    __device__ __forceinline__ uint32_t window_base(int x0, int y0) const
+    {
+        return ((uint32_t)y0 * (uint32_t)sx + (uint32_t)x0) * 16u;
+    }
+    __device__ __forceinline__ float window_sample(Plane p, uint32_t lane_off, uint32_t base) const
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p, lane_off, base, 0));
+    }
     __device__ __forceinline__ uint32_t col_index(int i) const { return (uint32_t)min(max(i, 0), sx - 1) * 16u; }
     __device__ __forceinline__ uint32_t row_index(int j) const
     {
@@ -808,10 +881,16 @@ __device__ __forceinline__ void scan_load_tile(const VolDesc& d, const uint16_t*
 // segment, y, x, z) order, so that consecutive items are consecutive slices
 // of one block column.  The last level-4 tile and the last column tile write
 // the totals, to device memory and to the host-visible copy.  The last
-// workgroup to have read the tile sums clears them for the next build.
+// workgroup to have read the tile sums clears them for the next build; it
+// is found in two steps, workgroup b counting out in group b mod
+// kScanGroups and the last of each group in the top counter, because
+// atomics on one address serialise (about 12 ns each: 512 workgroups on one
+// counter kept the last one waiting ~6 us).
+constexpr uint32_t kScanGroups = 16;
 struct ScanState {
-    uint32_t done;            // workgroups done with the tile sums
-    uint32_t pad[3];
+    uint32_t done;                      // groups whose workgroups are all done
+    uint32_t pad[15];                   // (the group counters on their own 64-byte line)
+    uint32_t group_done[kScanGroups];   // workgroups of each group done with the tile sums
 };
 
 __device__ __forceinline__ void scan_body(
@@ -848,11 +927,6 @@ __device__ __forceinline__ void scan_body(
         else mine.y += chunks_of(c[k]);
     }
     pre = block_reduce_u2(pre, sh);
-    // This workgroup has read every tile sum it needs (the reductions above
-    // consumed them): count it out now, so that the round trip overlaps the
-    // scan.  The last workgroup to do so clears the sums for the next build.
-    bool last = false;
-    if (threadIdx.x == 0) last = atomicAdd(&state->done, 1u) == ntiles - 1;
     uint2 inc = mine;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -864,6 +938,21 @@ __device__ __forceinline__ void scan_body(
     }
     if (lane == 63) wsum[w] = inc;
     __syncthreads();
+    // This workgroup has read every tile sum and entry it needs (the scans
+    // above consumed them): count it out now, so that the round trip overlaps
+    // the scatter (issued after the loads, it holds up no wait for them: the
+    // vector memory counter completes in order).  The last workgroup to do so
+    // clears the sums for the next build.  The address goes through a vector
+    // register so that the compiler keeps the atomic as issued instead of
+    // combining it over the wave, which reads its result at once.
+    const uint32_t grp = tile % kScanGroups;
+    uint32_t gcount = 0u;
+    if (threadIdx.x == 0) {
+        uint64_t a = (uint64_t)&state->group_done[grp];
+        asm volatile("" : "+v"(a));
+        gcount = __hip_atomic_fetch_add((__attribute__((address_space(1))) uint32_t*)a, 1u, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    }
     // A volume's first level-4 tile knows its vertex base, its last one the
     // end; the last tile of each sequence knows the grand totals (the sums
     // of the tiles before it plus its own).
@@ -925,7 +1014,16 @@ __device__ __forceinline__ void scan_body(
             for (uint32_t ch = 0; ch * kEmitChunk < c[k]; ++ch) items[off.y++] = make_uint2(key, yv | ch);
         }
     }
-    if (threadIdx.x == 0) s_last = last;
+    if (threadIdx.x == 0) {
+        bool last = false;
+        const uint32_t gsize = (ntiles - grp + kScanGroups - 1) / kScanGroups;
+        if (gcount == gsize - 1) {
+            // every workgroup of the group has counted out
+            __hip_atomic_store(&state->group_done[grp], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            last = atomicAdd(&state->done, 1u) == min(ntiles, kScanGroups) - 1;
+        }
+        s_last = last;
+    }
     __syncthreads();
     if (s_last) {
         for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) tile_sums[i] = make_uint2(0, 0);
@@ -1017,26 +1115,67 @@ __device__ __forceinline__ void block_origin(const Geom& g, uint32_t key, int& x
     y0 = Y0 & (g.tsy2 - 1);
 }
 
-// Loads planes p_lo..p_hi (1 to 3) of the 18x18 window at (x0, y0) into r.
+// Fields with unclamped window access (kWindows).
+template <class F, class = void>
+struct has_windows {
+    static constexpr bool value = false;
+};
 template <class F>
-// Loads planes zfirst .. zfirst + n - 1 (n = 0..3, wave-uniform) of the
-// 18x18 window at (x0, y0) into r[0 .. n-1].
-__device__ __forceinline__ void emit_fetch(const F& field, int x0, int y0, int zfirst, int n, int lane,
-                                           float (&r)[3][kPlaneLoads])
+struct has_windows<F, decltype((void)F::kWindows, void())> {
+    static constexpr bool value = F::kWindows;
+};
+
+// Per lane, the window offsets of its kPlaneLoads samples of an 18x18
+// window (fields with window access; a volume's own, as they scale with
+// its row length).
+template <class F>
+__device__ __forceinline__ void emit_window_lanes(const F& field, int lane, uint32_t (&wl)[kPlaneLoads])
 {
-    uint32_t idx[kPlaneLoads];
 #pragma unroll
     for (int k = 0; k < kPlaneLoads; ++k) {
-        const int i = min(lane + 64 * k, kBrickPlane - 1);
-        const int ky = i / kBrick, kx = i - ky * kBrick;
-        idx[k] = field.lane_index(x0 + kx, y0 + ky);
+        if constexpr (has_windows<F>::value) {
+            const int i = min(lane + 64 * k, kBrickPlane - 1);
+            const int ky = i / kBrick, kx = i - ky * kBrick;
+            wl[k] = field.window_lane(kx, ky);
+        } else {
+            wl[k] = 0u;
+        }
+    }
+}
+
+// Loads planes zfirst .. zfirst + n - 1 (n = 0..3, wave-uniform) of the
+// 18x18 window at (x0, y0) into r[0 .. n-1].  A window inside the lattice's
+// planes (all but the last block row and column) is read through the lane
+// offsets wl and a scalar corner offset; others clamp each sample.
+template <class F>
+__device__ __forceinline__ void emit_fetch(const F& field, int x0, int y0, int zfirst, int n, int lane,
+                                           const uint32_t (&wl)[kPlaneLoads], float (&r)[3][kPlaneLoads])
+{
+    uint32_t idx[kPlaneLoads];
+    uint32_t base = 0u;
+    bool inside = false;
+    if constexpr (has_windows<F>::value) inside = field.window_inside(x0, y0, kBrick);
+    if (inside) {
+        if constexpr (has_windows<F>::value) base = field.window_base(x0, y0);
+#pragma unroll
+        for (int k = 0; k < kPlaneLoads; ++k) idx[k] = wl[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < kPlaneLoads; ++k) {
+            const int i = min(lane + 64 * k, kBrickPlane - 1);
+            const int ky = i / kBrick, kx = i - ky * kBrick;
+            idx[k] = field.lane_index(x0 + kx, y0 + ky);
+        }
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         if (q < n) {
             const typename F::Plane pl = field.plane(zfirst + q);
 #pragma unroll
-            for (int k = 0; k < kPlaneLoads; ++k) r[q][k] = field.sample(pl, idx[k]);
+            for (int k = 0; k < kPlaneLoads; ++k) {
+                if constexpr (has_windows<F>::value) r[q][k] = field.window_sample(pl, idx[k], base);
+                else r[q][k] = field.sample(pl, idx[k]);
+            }
         }
     }
 }
@@ -1060,6 +1199,8 @@ __device__ __forceinline__ void emit_store(EmitWaveShared& ws, int zfirst, int n
 // otherwise every item belongs to vols[0].  mk(d) makes the field of
 // volume descriptor d.  nitems: the item total of the scan (the kernel loads
 // it before its LDS set-up, so that the two round trips overlap).
+constexpr uint32_t kEmitSkew = 13;   // share skew of emission workgroups by dispatch order, in 64ths
+
 template <int MODE, bool BATCH, class MK, class CB = NoVertexCallback>
 __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, const MK& mk, float T,
                                           const uint2* __restrict__ items, const uint32_t* __restrict__ l4_off,
@@ -1078,10 +1219,29 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     // of the same block column (or the next chunk of the same block), whose
     // planes are already staged
     const uint32_t t = (nwg & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (nwg >> 3) + (blockIdx.x >> 3);
-    const uint64_t nwaves = (uint64_t)nwg * kEmitWaves;
-    const uint64_t wlog = (uint64_t)t * kEmitWaves + wv;
-    uint32_t it = __builtin_amdgcn_readfirstlane((uint32_t)(wlog * nitems / nwaves));
-    const uint32_t iend = __builtin_amdgcn_readfirstlane((uint32_t)((wlog + 1) * nitems / nwaves));
+    const uint64_t wlog = (uint64_t)t * kEmitWaves + wv;   // (per-wave traces)
+    // Items of an XCD: one eighth of the run, in order.  Within the XCD the
+    // workgroups are dispatched in order q = 0, 1, ... and the SIMDs issue
+    // from their oldest waves first, so the workgroups dispatched first
+    // progress fastest; workgroup q gets a share of the XCD's items falling
+    // linearly from 1 + a to 1 - a (a = kEmitSkew / 64) so that they finish
+    // together.  Its four waves split the share evenly.
+    const uint32_t nx = (nwg & 7u) ? 1u : 8u;
+    const uint32_t x = (nwg & 7u) ? 0u : (blockIdx.x & 7u);
+    const uint32_t q = (nwg & 7u) ? blockIdx.x : (blockIdx.x >> 3);
+    const uint32_t nq = nwg / nx;
+    uint32_t it, iend;
+    {
+        const uint64_t lo = (uint64_t)x * nitems / nx, hi = (uint64_t)(x + 1) * nitems / nx;
+        // cumulative share of workgroups 0 .. k-1, times 64 (nq - 1) nq
+        auto cum = [&](uint64_t k) -> uint64_t {
+            return nq > 1 ? k * (64u + kEmitSkew) * (nq - 1) - (uint64_t)kEmitSkew * k * (k - (k > 0)) : 64u * k;
+        };
+        const uint64_t den = nq > 1 ? 64ull * (nq - 1) * nq : 64ull;
+        const uint64_t s0 = lo + (hi - lo) * cum(q) / den, s1 = lo + (hi - lo) * cum(q + 1) / den;
+        it = __builtin_amdgcn_readfirstlane((uint32_t)(s0 + (s1 - s0) * wv / kEmitWaves));
+        iend = __builtin_amdgcn_readfirstlane((uint32_t)(s0 + (s1 - s0) * (wv + 1) / kEmitWaves));
+    }
     const bool has_items = it < iend;
 
     EmitWaveShared& ws = sh.w[wv];
@@ -1109,11 +1269,19 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     int x0, y0, z;
     block_origin(dn.g, cur.x, x0, y0, z);
     int nnew = NP;                      // planes the item needs that are not staged yet
-    emit_fetch(mk(dn), x0, y0, z, NP, lane, r);
+    uint32_t wl[kPlaneLoads];           // window lane offsets of the volume fetched last
+    emit_window_lanes(mk(dn), lane, wl);
+    emit_fetch(mk(dn), x0, y0, z, NP, lane, wl, r);
     sh.cases[threadIdx.x] = case_word;
     sh.lut[threadIdx.x] = (uint8_t)(case_word & 0xfu);
     __syncthreads();
     if (!has_items) return;
+    // The first item's planes have landed before the loop starts, so that at
+    // the loop's head only a previous iteration's loads are pending, always
+    // followed by that iteration's vertex stores: the head's wait then
+    // leaves those stores in flight (the vector-memory counter covers loads
+    // and stores alike and completes in order).
+    __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0)
 
     // Per lane, the vertex counts and first vertices of its four cells and
     // the block's vertex total, kept from item to item: a further chunk of
@@ -1123,8 +1291,19 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     uint32_t cc[4] = {0u, 0u, 0u, 0u}, ex[4] = {0u, 0u, 0u, 0u}, blk_total = 0u;
     bool reuse = false;
     for (;;) {
+        // The SIMD issues from the oldest ready wave first, so the waves
+        // dispatched last on a SIMD fall behind and end up running their
+        // item chains alone; ranking waves by the items they have left keeps
+        // their progress even and the SIMDs busy to the end.
+        {
+            const uint32_t rem = iend - it;   // items left, this one included
+            if (rem >= 4u) __builtin_amdgcn_s_setprio(3);
+            else if (rem == 3u) __builtin_amdgcn_s_setprio(2);
+            else if (rem == 2u) __builtin_amdgcn_s_setprio(1);
+            else __builtin_amdgcn_s_setprio(0);
+        }
         const uint32_t vlo = item_chunk_first(cur.y), cnt = item_block_count(cur.y);
-        const uint32_t base = obase;
+        const uint32_t base = __builtin_amdgcn_readfirstlane(obase);
         const int cz = z, cx0 = x0, cy0 = y0;
         const uint32_t cvol = BATCH ? item_volume(cur.y) : 0u;
         const VolDesc dc = dn;          // this item's volume
@@ -1139,13 +1318,20 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
         bool reuse_next = false;
         if (more) {
             reuse_next = nxt.x == cur.x && (!BATCH || item_volume(nxt.y) == cvol);
-            if constexpr (BATCH) dn = load_uniform(vols + item_volume(nxt.y));
-            obase = load_uniform(l4_off + dn.l4_base + nxt.x);
+            // (the next descriptor is a scalar load, only when the volume
+            // changes; the offset and the entry after it go through the
+            // vector memory path, see prefetch_u32)
+            if constexpr (BATCH)
+                if (item_volume(nxt.y) != cvol) {
+                    dn = load_uniform(vols + item_volume(nxt.y));
+                    emit_window_lanes(mk(dn), lane, wl);
+                }
+            obase = prefetch_u32(l4_off + dn.l4_base + nxt.x);
             block_origin(dn.g, nxt.x, x0, y0, z);
             const bool column = x0 == cx0 && y0 == cy0 && (!BATCH || item_volume(nxt.y) == cvol);
             nnew = (column && z == cz) ? 0 : ((column && z == cz + 1) ? 1 : NP);
-            emit_fetch(mk(dn), x0, y0, z + NP - nnew, nnew, lane, r);
-            after = load_uniform(items + min(it + 2, iend - 1));
+            emit_fetch(mk(dn), x0, y0, z + NP - nnew, nnew, lane, wl, r);
+            after = prefetch_u2(items + min(it + 2, iend - 1));
         }
 
         // plane ring: plane cz + a sits in slot (cz + a) mod 3
@@ -1234,7 +1420,9 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
             Site s;
             s.c = ws.vcell[v - vlo];
             s.code = ws.code[s.c];
-            s.slot = v - ws.cexc[s.c];
+            // (clamped: a batch past the chunk's end, which stores nothing,
+            // still indexes the binary-normal table in bounds)
+            s.slot = min(v - ws.cexc[s.c], 15u);
             s.e = (uint32_t)(cases[s.code] >> (4 + 4 * s.slot)) & 0xfu;
             // edge e = 4 axis + s0 + 2 s1 -> corner offset (dx, dy, dz) of
             // its first end and the axis, bits 0..4 of a 5-bit entry
@@ -1333,15 +1521,21 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                 st[1] = (v2f){o[2], o[3]};
                 st[2] = (v2f){o[4], o[5]};
             }
-            const uint32_t start = base + v0;   // first output vertex of the batch
-            const uint32_t nval = min(vhi - v0, 64u);
-            const uint32_t nout = capacity > start ? min(nval, capacity - start) : 0u;
+            // (wave-uniform values, in scalar registers for the resource)
+            const uint32_t start = __builtin_amdgcn_readfirstlane(base + v0);   // first output vertex of the batch
+            const uint32_t nval = vhi > v0 ? min(vhi - v0, 64u) : 0u;
+            const uint32_t nout = __builtin_amdgcn_readfirstlane(capacity > start ? min(nval, capacity - start) : 0u);
+            // the batch's nout vertices as a buffer range: pairs past it are
+            // dropped by the range check, so every store issues (no branch)
+            const __amdgpu_buffer_rsrc_t ob =
+                __builtin_amdgcn_make_buffer_rsrc((void*)(out + (size_t)start * 6), (short)0, (int)(nout * 24u),
+                                                  0x00020000);
             wave_lds_sync();
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const uint32_t pr = 64u * k + lane;   // float pair: vertex pr / 3
-                const v2f val = ((const v2f*)ws.ostage)[pr];
-                if (pr / 3u < nout) __builtin_nontemporal_store(val, (v2f*)(out + (size_t)start * 6) + pr);
+                const v2u val = ((const v2u*)ws.ostage)[pr];
+                __builtin_amdgcn_raw_buffer_store_b64(val, ob, pr * 8u, 0, kStreamLoad);
             }
             wave_lds_sync();
             if (out_ce && v0 + (uint32_t)lane < vhi && start + lane < capacity) {
@@ -1376,18 +1570,23 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
                 }
             }
         } else {
-            for (uint32_t v0 = vlo; v0 < vhi; v0 += 64) {
-                const Site s = site_of(min(v0 + (uint32_t)lane, vhi - 1u));
+            // at least one batch (an item holds a vertex), so that every
+            // path from the next item's plane loads to their use issues
+            // stores, and the wait for those loads leaves the stores in flight
+            uint32_t v0 = vlo;
+            do {
+                const Site s = site_of(min(v0 + (uint32_t)lane, max(vhi, vlo + 1u) - 1u));
                 float o[6];
                 eval(s, o);
                 store_batch(v0, s, o);
-            }
+                v0 += 64;
+            } while (v0 < vhi);
         }
         if (!more) break;
         ++it;
         reuse = reuse_next;
         cur = nxt;
-        nxt = after;
+        nxt = uniform_u2(after);
     }
 }
 
