@@ -369,7 +369,7 @@ struct HPMCHistoPyramid {
     hpmc_host::Plan plan;
     hpmc_host::Kernel k_classify, k_emit, k_codes;
     hpmc_host::Kernel k_classify_small;   // grids with blocks narrower than 16 cells
-    hipEvent_t ev_totals = nullptr;  // after the scan: totals are in pinned memory
+    hipEvent_t ev_totals = nullptr;  // recorded by a count: the totals are in pinned memory once it completes
     hpmc_host::AdjointScratch adjoint;
     bool built = false;
     bool count_valid = false;
@@ -837,6 +837,8 @@ bool wait_count(HPMCHistoPyramid* h)
         h->count = 0;
         return true;
     }
+    // everything queued so far on the stream, the last build's scan included
+    HIP_TRY(h, hipEventRecord(h->ev_totals, h->stream));
     HIP_TRY(h, hipEventSynchronize(h->ev_totals));
     h->count = h->plan.h_totals->vertices;
     h->count_valid = true;
@@ -1036,12 +1038,10 @@ void HPMCbuildHistopyramid(struct HPMCHistoPyramid* h, float threshold)
         h->broken = true;
         return;
     }
-    // the scan writes the totals straight into the pinned host copy
-    if (hipEventRecord(h->ev_totals, h->stream) != hipSuccess) {
-        fail(h, HPMC_ERR_HIP, "failed to enqueue the count readback");
-        h->broken = true;
-        return;
-    }
+    // The scan writes the totals straight into the pinned host copy; the
+    // event a count waits on is recorded when the count is asked for (an
+    // event recorded here, between the scan and the extraction, held the
+    // extraction back by ~5.6 us).
     h->built = true;
     ok(h);
 }
