@@ -812,11 +812,9 @@ __device__ __forceinline__ void classify_small_body(const F& field, const Geom& 
 __device__ __forceinline__ uint2 block_reduce_u2(uint2 v, uint2* sh)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        v.x += __shfl_xor(v.x, o);
-        v.y += __shfl_xor(v.y, o);
-    }
+    // wave totals by DPP scans (no LDS round trips; every lane active)
+    v.x = (uint32_t)__builtin_amdgcn_readlane(wave_scan_u32<false>(v.x), 63);
+    v.y = (uint32_t)__builtin_amdgcn_readlane(wave_scan_u32<false>(v.y), 63);
     if (lane == 0) sh[w] = v;
     __syncthreads();
     uint2 r = make_uint2(0, 0);
@@ -927,15 +925,8 @@ __device__ __forceinline__ void scan_body(
         else mine.y += chunks_of(c[k]);
     }
     pre = block_reduce_u2(pre, sh);
-    uint2 inc = mine;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t ax = __shfl_up(inc.x, o), ay = __shfl_up(inc.y, o);
-        if (lane >= o) {
-            inc.x += ax;
-            inc.y += ay;
-        }
-    }
+    // inclusive wave scans by DPP (every lane active)
+    const uint2 inc = make_uint2(wave_scan_u32<false>(mine.x), wave_scan_u32<false>(mine.y));
     if (lane == 63) wsum[w] = inc;
     __syncthreads();
     // This workgroup has read every tile sum and entry it needs (the scans
@@ -1218,8 +1209,6 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
     // contiguous run of items: consecutive items are mostly the next slice
     // of the same block column (or the next chunk of the same block), whose
     // planes are already staged
-    const uint32_t t = (nwg & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (nwg >> 3) + (blockIdx.x >> 3);
-    const uint64_t wlog = (uint64_t)t * kEmitWaves + wv;   // (per-wave traces)
     // Items of an XCD: one eighth of the run, in order.  Within the XCD the
     // workgroups are dispatched in order q = 0, 1, ... and the SIMDs issue
     // from their oldest waves first, so the workgroups dispatched first
