@@ -49,6 +49,39 @@ def test_set_stream_orders_new_stream_after_old():
     ref.close()
 
 
+def test_count_waits_for_the_last_build_whatever_was_queued_after_it():
+    # The count's event is recorded when the count is asked for: it must see
+    # the last build's totals after an extraction queued behind the build,
+    # after two builds in a row at different thresholds, and after the
+    # stream was switched between the build and the count.
+    n = 96
+    vol = torch.from_numpy(workloads.upsampled_noise(n)).cuda()
+    counts = {}
+    for iso_value in (0.4, 0.6):
+        r = IsoSurface((n, n, n))
+        r.set_volume(vol)
+        r.build(iso_value)
+        counts[iso_value] = r.count()
+        r.close()
+    assert counts[0.4] != counts[0.6] and min(counts.values()) > 0
+    a, b = torch.cuda.Stream(), torch.cuda.Stream()
+    iso = IsoSurface((n, n, n), stream=a)
+    iso.set_volume(vol)
+    out = torch.empty((max(counts.values()), 6), device="cuda")
+    iso.bind_output(out)
+    iso.build(0.4)
+    iso.extract()                                      # queued behind the build, no count yet
+    assert iso.count() == counts[0.4]
+    iso.build(0.4)
+    iso.build(0.6)                                     # two builds, no count in between
+    assert iso.count() == counts[0.6]
+    iso.build(0.4)
+    hpmc_amd._check(hpmc_amd.lib().HPMCsetStream(iso.h, hpmc_amd._stream_ptr(b)), iso.h)
+    assert iso.count() == counts[0.4]                  # recorded on B, which waits for A
+    torch.cuda.synchronize()
+    iso.close()
+
+
 def test_null_volume_after_build_is_a_field_error():
     lattice = (40, 36, 33)
     f = smooth_noise(lattice, 5)
