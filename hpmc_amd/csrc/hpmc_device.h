@@ -846,13 +846,43 @@ __device__ __forceinline__ void scan_load16_u16(const uint16_t* __restrict__ cc,
     }
 }
 
-// The 16 level-4 counts of thread t of a tile of Morton keys, gathered from
-// the column-major counts (all 16 loads in flight together).
+// The 16 level-4 counts of keys base .. base+15 (base a multiple of 16),
+// gathered from the column-major counts, all 16 loads in flight together.
+// The keys are a 4x4 square of blocks (key bits 0-3 = x0, y0, x1, y1).
+// When slice tiles are at least 4 blocks across both ways the square lies
+// in one slice tile: its slice, z segment and first count are decoded once
+// and the 16 loads step by constant block offsets (a key decoded per load
+// took ~75 instructions, most of the scan's time).  Counts past the grid
+// read entry 0 and are dropped.
 __device__ __forceinline__ void scan_gather16(const Geom& g, const uint16_t* __restrict__ colcnt, uint32_t n,
                                               uint32_t base, uint32_t (&c)[kScanPer])
 {
+    static_assert(kScanPer == 16, "a thread's keys are a 4x4 block square");
+    const bool square = g.tsx2_log2 - g.blog2 >= 2 && g.tsy2_log2 - g.blog2 >= 2;
+    if (!square || base + kScanPer > n) {
 #pragma unroll
-    for (int k = 0; k < kScanPer; ++k) c[k] = (base + k < n) ? key_count(g, colcnt, base + k) : 0u;
+        for (int k = 0; k < kScanPer; ++k) c[k] = (base + k < n) ? key_count(g, colcnt, base + k) : 0u;
+        return;
+    }
+    const int X = (int)morton_compact(base) << g.blog2, Y = (int)morton_compact(base >> 1) << g.blog2;
+    const int tx = X >> g.tsx2_log2, ty = Y >> g.tsy2_log2;
+    const int z = (ty << g.tiles_x_log2) + tx;
+    const int bx = (X & (g.tsx2 - 1)) >> g.blog2, by = (Y & (g.tsy2 - 1)) >> g.blog2;
+    const bool zin = tx < g.tiles_x && z < g.cz;
+    const uint32_t first =
+        ((((uint32_t)(z >> g.zseg_log2) * (uint32_t)g.nby + (uint32_t)by) * (uint32_t)g.nbx + (uint32_t)bx) *
+         (uint32_t)kColStride) + (uint32_t)(z & (g.zseg - 1));
+    const uint32_t row = (uint32_t)g.nbx * (uint32_t)kColStride;
+    uint32_t v[kScanPer];
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const int dx = (k & 1) | ((k >> 1) & 2), dy = ((k >> 1) & 1) | ((k >> 2) & 2);
+        const bool in = zin && bx + dx < g.nbx && by + dy < g.nby;
+        v[k] = colcnt[in ? first + (uint32_t)dy * row + (uint32_t)(dx * kColStride) : 0u];
+        c[k] = in;
+    }
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) c[k] = c[k] ? v[k] : 0u;
 }
 
 // Scan tiles of a build: [0, n_l4t) are level-4 tiles (4096 Morton keys of
@@ -996,14 +1026,26 @@ __device__ __forceinline__ void scan_body(
         const uint32_t rest = wi / (uint32_t)g.nbx;
         const int by = (int)(rest % (uint32_t)g.nby);
         const int pz = (int)(rest / (uint32_t)g.nby);
+        // The thread's slices z0 + k: z0 is a multiple of 16, or of 8 for
+        // 8-slice segments, whose slots past 8 are empty.  With at least 16
+        // slices per row of slice tiles the run stays in one tile row, and
+        // key(z0 + k) = key(z0) | spread(k) << (2 * log2 of the blocks across
+        // a tile), spread(k) a constant of the unrolled loop.
+        const int z0 = pz * g.zseg + (int)(base % (uint32_t)kColStride);
+        const bool one_row = g.tiles_x_log2 >= 4;
+        const uint32_t key0 = block_key(g, bxb << g.blog2, by << g.blog2, z0);
+        const uint32_t kshift = 2u * (uint32_t)(g.tsx2_log2 - g.blog2);
+        auto scatter = [&](auto key_of) {
 #pragma unroll
-        for (int k = 0; k < kScanPer; ++k) {
-            if (!c[k]) continue;
-            const int zi = (int)(base % (uint32_t)kColStride) + k;
-            const uint32_t key = block_key(g, bxb << g.blog2, by << g.blog2, pz * g.zseg + zi);
-            const uint32_t yv = (c[k] << 4) | (vol << 16);
-            for (uint32_t ch = 0; ch * kEmitChunk < c[k]; ++ch) items[off.y++] = make_uint2(key, yv | ch);
-        }
+            for (int k = 0; k < kScanPer; ++k) {
+                if (!c[k]) continue;
+                const uint32_t key = key_of(k);
+                const uint32_t yv = (c[k] << 4) | (vol << 16);
+                for (uint32_t ch = 0; ch * kEmitChunk < c[k]; ++ch) items[off.y++] = make_uint2(key, yv | ch);
+            }
+        };
+        if (one_row) scatter([&](int k) { return key0 | (morton_spread((uint32_t)k) << kshift); });
+        else scatter([&](int k) { return block_key(g, bxb << g.blog2, by << g.blog2, z0 + k); });
     }
     if (threadIdx.x == 0) {
         bool last = false;
