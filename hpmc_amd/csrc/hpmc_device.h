@@ -1122,6 +1122,21 @@ constexpr int emit_planes() { return MODE == kEmitContinuous ? 3 : 2; }
 // lanes, no tail); planes sit kPlaneStride floats apart so a sample and its
 // +z neighbour are one ds_read2st64 apart.
 constexpr int kPlaneStride = 64 * kPlaneLoads;   // 384
+// Row stride of a staged window in LDS.  A lane's 2x2 quad of cells sits at
+// (2 cx, 2 cy), cx and cy the even and odd bits of the lane; with rows 18
+// floats apart the 64 lanes' reads of one corner fall 3-4 deep into one
+// bank, with 20 two deep.  Load k of lane l fetches the sample at padded
+// position p = l + 64 k, (p / 20, p % 20), so the staging writes stay
+// contiguous; the two padding columns of a row read a clamped sample that
+// nothing uses.
+constexpr int kBrickStride = 20;
+static_assert(kBrick * kBrickStride <= kPlaneStride, "a padded window fits its slot");
+__device__ __forceinline__ void brick_lane_pos(int lane, int k, int& kx, int& ky)
+{
+    const int p = min(lane + 64 * k, kBrick * kBrickStride - 1);
+    ky = p / kBrickStride;
+    kx = min(p - ky * kBrickStride, kBrick - 1);
+}
 
 struct EmitWaveShared {
     float plane[3][kPlaneStride];  // planes z, z+1, z+2 of the item
@@ -1167,8 +1182,8 @@ __device__ __forceinline__ void emit_window_lanes(const F& field, int lane, uint
 #pragma unroll
     for (int k = 0; k < kPlaneLoads; ++k) {
         if constexpr (has_windows<F>::value) {
-            const int i = min(lane + 64 * k, kBrickPlane - 1);
-            const int ky = i / kBrick, kx = i - ky * kBrick;
+            int kx, ky;
+            brick_lane_pos(lane, k, kx, ky);
             wl[k] = field.window_lane(kx, ky);
         } else {
             wl[k] = 0u;
@@ -1195,8 +1210,8 @@ __device__ __forceinline__ void emit_fetch(const F& field, int x0, int y0, int z
     } else {
 #pragma unroll
         for (int k = 0; k < kPlaneLoads; ++k) {
-            const int i = min(lane + 64 * k, kBrickPlane - 1);
-            const int ky = i / kBrick, kx = i - ky * kBrick;
+            int kx, ky;
+            brick_lane_pos(lane, k, kx, ky);
             idx[k] = field.lane_index(x0 + kx, y0 + ky);
         }
     }
@@ -1382,7 +1397,7 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
             for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
                 for (int dx = 0; dx < 3; ++dx) {
-                    const int i = (qy + dy) * kBrick + qx + dx;
+                    const int i = (qy + dy) * kBrickStride + qx + dx;
                     bits0 |= (P0[i] < T ? 1u : 0u) << (dx + 3 * dy);
                     bits1 |= (P1[i] < T ? 1u : 0u) << (dx + 3 * dy);
                 }
@@ -1472,7 +1487,7 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
         };
         // The staged samples of continuous emission: the edge's ends a and b
         // and their forward neighbours na = f(a + e_q), nb = f(b + e_q).
-        // Sample (x, y, plane) sits at x + 18 y + 384 slot of the staged
+        // Sample (x, y, plane) sits at x + 20 y + 384 slot of the staged
         // brick; a and b lie in planes 0 or 1, their +z neighbours in planes
         // 1 or 2 (an edge along z starts in plane 0).
         struct EdgeSamples {
@@ -1482,17 +1497,17 @@ __device__ __forceinline__ void emit_body(const VolDesc* __restrict__ vols, cons
             const int bx_ = s.ax + (s.axis == 0), by_ = s.ay + (s.axis == 1), bz_ = s.az + (s.axis == 2);
             const float* Pw = ws.plane[0];
             const int o0 = s0 * kPlaneStride, o1 = s1 * kPlaneStride, o2 = s2 * kPlaneStride;
-            const int ia = (int)__umul24((uint32_t)s.ay, (uint32_t)kBrick) + s.ax;
-            const int ib = (int)__umul24((uint32_t)by_, (uint32_t)kBrick) + bx_;
+            const int ia = (int)__umul24((uint32_t)s.ay, (uint32_t)kBrickStride) + s.ax;
+            const int ib = (int)__umul24((uint32_t)by_, (uint32_t)kBrickStride) + bx_;
             const int pa = s.az ? o1 : o0, pa1 = s.az ? o2 : o1, pb = bz_ ? o1 : o0, pb1 = bz_ ? o2 : o1;
             EdgeSamples q;
             q.va = Pw[ia + pa];
             q.vb = Pw[ib + pb];
             q.nax = Pw[ia + pa + 1];
-            q.nay = Pw[ia + pa + kBrick];
+            q.nay = Pw[ia + pa + kBrickStride];
             q.naz = Pw[ia + pa1];
             q.nbx = Pw[ib + pb + 1];
-            q.nby = Pw[ib + pb + kBrick];
+            q.nby = Pw[ib + pb + kBrickStride];
             q.nbz = Pw[ib + pb1];
             return q;
         };
