@@ -48,10 +48,10 @@ HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume_batch, HPMC_VOLUME_FIELD, hpmc::kEmitCo
 
 extern "C" __global__ __launch_bounds__(hpmc::kScanThreads) void hpmc_scan(
     const hpmc::VolDesc* vols, const uint32_t* tile_vol, uint32_t n_l4t, uint32_t n_colt, const uint16_t* colcnt,
-    uint2* tile_sums, hpmc::ScanState* state, uint32_t* l4_off, uint2* items, hpmc::Totals* totals,
-    hpmc::Totals* host_totals, hpmc::VolCount* vol_counts, hpmc::VolCount* host_vol_counts)
+    const uint2* tile_sums, uint32_t* l4_off, uint2* items, hpmc::Totals* totals, hpmc::Totals* host_totals,
+    hpmc::VolCount* vol_counts, hpmc::VolCount* host_vol_counts)
 {
-    hpmc::scan_body(vols, tile_vol, n_l4t, n_colt, colcnt, tile_sums, state, l4_off, items, totals, host_totals,
+    hpmc::scan_body(vols, tile_vol, n_l4t, n_colt, colcnt, tile_sums, l4_off, items, totals, host_totals,
                     vol_counts, host_vol_counts);
 }
 
@@ -127,23 +127,49 @@ extern "C" __global__ __launch_bounds__(256) void hpmc_fill_f32(float* p, uint64
 }
 
 // Start of a no-allocation build: the volume's descriptor into its scratch
-// slot (passed by value, so a captured graph carries it), the scan tile sums
-// and the scan's counter cleared.
+// slot (passed by value, so a captured graph carries it) and the scan tile
+// sums cleared.
 extern "C" __global__ __launch_bounds__(256) void hpmc_noalloc_begin(hpmc::VolDesc d, hpmc::VolDesc* dst,
-                                                                     uint2* tile_sums, uint32_t ntiles,
-                                                                     hpmc::ScanState* state)
+                                                                     uint2* tile_sums, uint32_t ntiles)
 {
-    if (threadIdx.x == 0) {
-        *dst = d;
-        state->done = 0u;
-    }
-    if (threadIdx.x < hpmc::kScanGroups) state->group_done[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) *dst = d;
     for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) tile_sums[i] = make_uint2(0u, 0u);
+}
+
+// Clears the scan tile sums before a build that follows no extraction.
+extern "C" __global__ __launch_bounds__(256) void hpmc_clear_sums(uint2* tile_sums, uint32_t ntiles)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ntiles; i += gridDim.x * blockDim.x)
+        tile_sums[i] = make_uint2(0u, 0u);
 }
 
 #include "hpmc_plan.h"
 
 namespace hpmc_host {
+
+// Before a build's classification: the extraction of the previous build
+// cleared the scan tile sums that classification adds into; clear them here
+// when no extraction followed it.  In a graph being captured (which may be
+// replayed without an extraction in between) the clear is part of the graph
+// unless the capture also holds the extraction (whose replays clear them,
+// the graph's launch clearing them first if need be).  Marks the sums as the
+// new build's.
+hipError_t sums_prepare(Plan& p, hipStream_t stream, bool extract_in_capture)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(stream, &cs);
+    if (e != hipSuccess) return e;
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    if (capturing ? !extract_in_capture : p.sums_dirty) {
+        const uint32_t n = p.n_l4t + p.n_colt;
+        hipLaunchKernelGGL(hpmc_clear_sums, dim3(std::min<uint32_t>((n + 255) / 256, 1024u)), dim3(256), 0, stream,
+                           p.tile_sums, n);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    p.sums_dirty = true;
+    return hipSuccess;
+}
 
 // Scratch of the adjoint, grown on demand: per lattice sample a cell code
 // and a first vertex (cells by their origin sample), followed by one flag
@@ -798,6 +824,7 @@ bool launch_classify(HPMCHistoPyramid* h, float T)
         }
         HIP_TRY(h, hpmc_host::plan_set_field(h->plan, 0, volume_ptr(h), h->stream));
     }
+    HIP_TRY(h, hpmc_host::sums_prepare(h->plan, h->stream, false));
     stage_mark(h, kStageClassify);
     // 8-bit volumes classify raw bytes against the equivalent byte threshold
     const float Tc = h->mode == kFieldVolume && h->unorm8 ? unorm8_threshold(T) : T;
@@ -826,6 +853,7 @@ bool launch_emit(HPMCHistoPyramid* h, float* out, hpmc::CellEdge* out_ce, uint32
     HIP_TRY(h, hpmc_host::plan_emit(h->plan, h->k_emit, h->threshold, out, out_ce, capacity, h->constants->d_cases,
                                     h->constants->d_bnorm, h->stream));
     HIP_TRY(h, hipGetLastError());
+    h->plan.sums_dirty = false;   // the extraction clears the tile sums
     stage_mark(h, kStageEmit);
     return true;
 }
@@ -1482,8 +1510,10 @@ int32_t batch_setup(HPMCBatch* b)
     return HPMC_OK;
 }
 
-int32_t batch_enqueue_build(HPMCBatch* b, float T)
+// extract_in_capture: the extraction is enqueued in the same graph capture.
+int32_t batch_enqueue_build(HPMCBatch* b, float T, bool extract_in_capture = false)
 {
+    BATCH_TRY(hpmc_host::sums_prepare(b->plan, b->stream, extract_in_capture));
     BATCH_TRY(hpmc_host::plan_classify(b->plan, hpmc_host::Kernel{(const void*)hpmc_classify_volume, nullptr}, T,
                                        b->constants->d_cases, b->stream));
     BATCH_TRY(hpmc_host::plan_scan(b->plan, (const void*)hpmc_scan, b->stream));
@@ -1498,6 +1528,7 @@ int32_t batch_enqueue_extract(HPMCBatch* b, float T, float* out, uint64_t capaci
     BATCH_TRY(hpmc_host::plan_emit(b->plan, hpmc_host::Kernel{(const void*)hpmc_emit_volume_batch, nullptr}, T, out,
                                    (hpmc::CellEdge*)cell_edge, cap, b->constants->d_cases, b->constants->d_bnorm,
                                    b->stream));
+    b->plan.sums_dirty = false;   // the extraction clears the tile sums
     return HPMC_OK;
 }
 
@@ -1654,8 +1685,10 @@ int32_t HPMCbatchCaptureGraph(struct HPMCBatch* b, float threshold, float* out, 
     BATCH_TRY(hipStreamSynchronize(b->stream));
     BATCH_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     b->last_threshold = threshold;
-    rc = batch_enqueue_build(b, threshold);
+    const bool dirty = b->plan.sums_dirty;   // (capture runs nothing)
+    rc = batch_enqueue_build(b, threshold, true);
     if (rc == HPMC_OK) rc = batch_enqueue_extract(b, threshold, out, capacity, cell_edge);
+    b->plan.sums_dirty = dirty;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc != HPMC_OK) {
@@ -1674,7 +1707,16 @@ int32_t HPMCbatchLaunchGraph(struct HPMCBatch* b)
 {
     if (!b) return fail(nullptr, HPMC_ERR_NULL, "NULL batch");
     if (!b->graph_exec || b->tainted) return fail(nullptr, HPMC_ERR_ARG, "no captured graph for this configuration");
+    // the graph's build adds into the tile sums that its extraction clears
+    // again: clear them first after a build that no extraction followed
+    if (b->plan.sums_dirty) {
+        const uint32_t n = b->plan.n_l4t + b->plan.n_colt;
+        hipLaunchKernelGGL(hpmc_clear_sums, dim3(std::min<uint32_t>((n + 255) / 256, 1024u)), dim3(256), 0,
+                           b->stream, b->plan.tile_sums, n);
+        BATCH_TRY(hipGetLastError());
+    }
     BATCH_TRY(hipGraphLaunch(b->graph_exec, b->stream));
+    b->plan.sums_dirty = false;
     // the replayed build is at the captured threshold: later extractions
     // and adjoints of this build use it
     b->last_threshold = b->graph_threshold;
@@ -1796,7 +1838,7 @@ int32_t HPMCextractVolumeNoAlloc(struct HPMCConstants* c, const float* volume, i
     (void)hpmc_host::plan_carve(p, base);
     const hipStream_t stream = (hipStream_t)hip_stream;
     hipLaunchKernelGGL(hpmc_noalloc_begin, dim3(1), dim3(256), 0, stream, desc, p.d_vols, p.tile_sums,
-                       p.n_l4t + p.n_colt, p.scan_state);
+                       p.n_l4t + p.n_colt);
     // lattices past the MALL stream through classification (as setup picks)
     const uint64_t bytes = 4ull * (uint64_t)x * (uint64_t)y * (uint64_t)z;
     const hpmc_host::Kernel k_cls{bytes >= (256ull << 20) ? (const void*)hpmc_classify_volume_stream

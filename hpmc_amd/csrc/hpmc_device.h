@@ -908,28 +908,18 @@ __device__ __forceinline__ void scan_load_tile(const VolDesc& d, const uint16_t*
 // work items -- one per kEmitChunk vertices of a block -- in (volume, z
 // segment, y, x, z) order, so that consecutive items are consecutive slices
 // of one block column.  The last level-4 tile and the last column tile write
-// the totals, to device memory and to the host-visible copy.  The last
-// workgroup to have read the tile sums clears them for the next build; it
-// is found in two steps, workgroup b counting out in group b mod
-// kScanGroups and the last of each group in the top counter, because
-// atomics on one address serialise (about 12 ns each: 512 workgroups on one
-// counter kept the last one waiting ~6 us).
-constexpr uint32_t kScanGroups = 16;
-struct ScanState {
-    uint32_t done;                      // groups whose workgroups are all done
-    uint32_t pad[15];                   // (the group counters on their own 64-byte line)
-    uint32_t group_done[kScanGroups];   // workgroups of each group done with the tile sums
-};
-
+// the totals, to device memory and to the host-visible copy.  The tile
+// sums stay as they are: the extraction that follows clears them for the
+// next build (or the host, before a build that follows no extraction), so
+// no workgroup waits to learn that it is the last.
 __device__ __forceinline__ void scan_body(
     const VolDesc* __restrict__ vols, const uint32_t* __restrict__ tile_vol, uint32_t n_l4t, uint32_t n_colt,
-    const uint16_t* __restrict__ colcnt, uint2* __restrict__ tile_sums, ScanState* __restrict__ state,
+    const uint16_t* __restrict__ colcnt, const uint2* __restrict__ tile_sums,
     uint32_t* __restrict__ l4_off, uint2* __restrict__ items, Totals* __restrict__ totals,
     Totals* __restrict__ host_totals, VolCount* __restrict__ vol_counts, VolCount* __restrict__ host_vol_counts)
 {
     __shared__ uint2 sh[16];
     __shared__ uint2 wsum[16];
-    __shared__ uint32_t s_last;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t ntiles = n_l4t + n_colt;
     const uint32_t tile = blockIdx.x;
@@ -959,21 +949,6 @@ __device__ __forceinline__ void scan_body(
     const uint2 inc = make_uint2(wave_scan_u32<false>(mine.x), wave_scan_u32<false>(mine.y));
     if (lane == 63) wsum[w] = inc;
     __syncthreads();
-    // This workgroup has read every tile sum and entry it needs (the scans
-    // above consumed them): count it out now, so that the round trip overlaps
-    // the scatter (issued after the loads, it holds up no wait for them: the
-    // vector memory counter completes in order).  The last workgroup to do so
-    // clears the sums for the next build.  The address goes through a vector
-    // register so that the compiler keeps the atomic as issued instead of
-    // combining it over the wave, which reads its result at once.
-    const uint32_t grp = tile % kScanGroups;
-    uint32_t gcount = 0u;
-    if (threadIdx.x == 0) {
-        uint64_t a = (uint64_t)&state->group_done[grp];
-        asm volatile("" : "+v"(a));
-        gcount = __hip_atomic_fetch_add((__attribute__((address_space(1))) uint32_t*)a, 1u, __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-    }
     // A volume's first level-4 tile knows its vertex base, its last one the
     // end; the last tile of each sequence knows the grand totals (the sums
     // of the tiles before it plus its own).
@@ -1046,21 +1021,6 @@ __device__ __forceinline__ void scan_body(
         };
         if (one_row) scatter([&](int k) { return key0 | (morton_spread((uint32_t)k) << kshift); });
         else scatter([&](int k) { return block_key(g, bxb << g.blog2, by << g.blog2, z0 + k); });
-    }
-    if (threadIdx.x == 0) {
-        bool last = false;
-        const uint32_t gsize = (ntiles - grp + kScanGroups - 1) / kScanGroups;
-        if (gcount == gsize - 1) {
-            // every workgroup of the group has counted out
-            __hip_atomic_store(&state->group_done[grp], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            last = atomicAdd(&state->done, 1u) == min(ntiles, kScanGroups) - 1;
-        }
-        s_last = last;
-    }
-    __syncthreads();
-    if (s_last) {
-        for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) tile_sums[i] = make_uint2(0, 0);
-        if (threadIdx.x == 0) state->done = 0u;
     }
 }
 
@@ -2401,9 +2361,12 @@ __device__ __forceinline__ void for_each_vertex(const TraversalView& v, const MK
     extern "C" __global__ __launch_bounds__(256) void NAME(                                   \
         const hpmc::VolDesc* vols, float T, const uint2* items, const uint32_t* l4_off,        \
         const hpmc::Totals* totals, float* out, hpmc::CellEdge* out_ce, uint32_t capacity,    \
-        const uint64_t* cases, const float4* bnorm)                                           \
+        const uint64_t* cases, const float4* bnorm, uint2* tile_sums, uint32_t ntiles)        \
     {                                                                                         \
         __shared__ hpmc::EmitShared sh;                                                       \
+        /* the build's scan tile sums, read by its scan only: cleared for the next build */   \
+        if (threadIdx.x == 0)                                                                 \
+            for (uint32_t i = blockIdx.x; i < ntiles; i += gridDim.x) tile_sums[i] = make_uint2(0u, 0u); \
         const uint32_t nitems = hpmc::load_uniform(&totals->items);                           \
         const uint64_t case_word = cases[threadIdx.x];   /* stored to LDS by emit_body */     \
         auto mk = [](const hpmc::VolDesc& dd) { return MAKE_FIELD; };                         \

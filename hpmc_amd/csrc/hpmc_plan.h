@@ -10,8 +10,9 @@
 //              slices per block column (whole 64-byte lines)
 //   l4_off     u32 per (volume, level-4 key): global offset of the block's
 //              first vertex in the concatenated output
-//   tile_sums  uint2 per scan tile, accumulated by classify (atomics) and
-//              cleared by the scan's last workgroup; scan_state its counter
+//   tile_sums  uint2 per scan tile, accumulated by classify (atomics), read
+//              by the scan and cleared by the extraction that follows (or
+//              before the next build, when none followed: sums_dirty)
 //   items      uint2 per emit work item (<= 15 per non-empty block slice)
 //   totals / vol_counts: grand totals and per-volume (base, count), also
 //              written straight into pinned host memory
@@ -173,7 +174,6 @@ struct Plan {
     uint16_t* colcnt = nullptr;
     uint32_t* l4_off = nullptr;
     uint2* tile_sums = nullptr;
-    hpmc::ScanState* scan_state = nullptr;
     uint2* items = nullptr;
     hpmc::Totals* d_totals = nullptr;
     hpmc::Totals* h_totals = nullptr;     // pinned
@@ -182,6 +182,9 @@ struct Plan {
     uint32_t* d_gather_tile0 = nullptr;   // first adjoint gather tile of each volume
     uint32_t* d_gather_vol = nullptr;     // volume of each adjoint gather tile
     uint64_t n_gather_tiles = 0;
+    // host view of the plan's stream: the tile sums hold a build's counts
+    // that no extraction has cleared yet
+    bool sums_dirty = false;
 };
 
 inline void plan_free(Plan& p)
@@ -192,7 +195,6 @@ inline void plan_free(Plan& p)
     if (p.colcnt) (void)hipFree(p.colcnt);
     if (p.l4_off) (void)hipFree(p.l4_off);
     if (p.tile_sums) (void)hipFree(p.tile_sums);
-    if (p.scan_state) (void)hipFree(p.scan_state);
     if (p.items) (void)hipFree(p.items);
     if (p.d_totals) (void)hipFree(p.d_totals);
     if (p.h_totals) (void)hipHostFree(p.h_totals);
@@ -367,8 +369,6 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
     PLAN_TRY(hipMalloc(&p.l4_off, p.n_l4 * sizeof(uint32_t)));
     PLAN_TRY(hipMalloc(&p.tile_sums, (size_t)(p.n_l4t + p.n_colt) * sizeof(uint2)));
     PLAN_TRY(hipMemset(p.tile_sums, 0, (size_t)(p.n_l4t + p.n_colt) * sizeof(uint2)));
-    PLAN_TRY(hipMalloc(&p.scan_state, sizeof(hpmc::ScanState)));
-    PLAN_TRY(hipMemset(p.scan_state, 0, sizeof(hpmc::ScanState)));
     PLAN_TRY(hipMalloc(&p.items, std::max<uint64_t>(p.n_items_max, 1) * sizeof(uint2)));
     PLAN_TRY(hipMalloc(&p.d_totals, sizeof(hpmc::Totals)));
     PLAN_TRY(hipMemset(p.d_totals, 0, sizeof(hpmc::Totals)));
@@ -398,7 +398,7 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
 }
 
 // Device bytes of a one-volume plan's arrays carved from one buffer (each
-// array 256-byte aligned): descriptor, scan state, totals, volume count,
+// array 256-byte aligned): descriptor, totals, volume count,
 // tile sums, column counts, block offsets, work items.
 inline uint64_t plan_carve(Plan& p, char* base)
 {
@@ -409,7 +409,6 @@ inline uint64_t plan_carve(Plan& p, char* base)
         return q;
     };
     p.d_vols = (hpmc::VolDesc*)take(sizeof(hpmc::VolDesc));
-    p.scan_state = (hpmc::ScanState*)take(sizeof(hpmc::ScanState));
     p.d_totals = (hpmc::Totals*)take(sizeof(hpmc::Totals));
     p.d_vcounts = (hpmc::VolCount*)take(sizeof(hpmc::VolCount));
     p.tile_sums = (uint2*)take((uint64_t)(p.n_l4t + p.n_colt) * sizeof(uint2));
@@ -494,15 +493,14 @@ inline hipError_t plan_scan(const Plan& p, const void* k_scan, hipStream_t strea
     const uint32_t* tv = p.nvol == 1 ? nullptr : p.d_tile_vol;
     uint32_t nl = p.n_l4t, nc = p.n_colt;
     const uint16_t* cc = p.colcnt;
-    uint2* ts = p.tile_sums;
-    hpmc::ScanState* st = p.scan_state;
+    const uint2* ts = p.tile_sums;
     uint32_t* lo = p.l4_off;
     uint2* it = p.items;
     hpmc::Totals* dt = p.d_totals;
     hpmc::Totals* ht = p.h_totals;
     hpmc::VolCount* dv = p.d_vcounts;
     hpmc::VolCount* hv = p.h_vcounts;
-    void* args[] = {&vols, &tv, &nl, &nc, &cc, &ts, &st, &lo, &it, &dt, &ht, &dv, &hv};
+    void* args[] = {&vols, &tv, &nl, &nc, &cc, &ts, &lo, &it, &dt, &ht, &dv, &hv};
     return hipLaunchKernel(k_scan, dim3(grid), dim3(hpmc::kScanThreads), args, 0, stream);
 }
 
@@ -519,7 +517,9 @@ inline hipError_t plan_emit(const Plan& p, const Kernel& k, float T, float* out,
     uint32_t cap = capacity;
     const uint64_t* cs = cases;
     const float4* bn = bnorm;
-    void* args[] = {&vols, &Ta, &it, &lo, &tt, &o, &oc, &cap, &cs, &bn};
+    uint2* ts = p.tile_sums;
+    uint32_t nts = p.n_l4t + p.n_colt;
+    void* args[] = {&vols, &Ta, &it, &lo, &tt, &o, &oc, &cap, &cs, &bn, &ts, &nts};
     return launch(k, (unsigned)p.emit_wgs, 256, args, stream);
 }
 

@@ -82,6 +82,64 @@ def test_count_waits_for_the_last_build_whatever_was_queued_after_it():
     iso.close()
 
 
+def test_scan_sums_survive_builds_without_extraction():
+    # Classification adds each build's counts into the scan's tile sums; the
+    # extraction that follows clears them, and a build that follows no
+    # extraction clears them first.  Any mix of builds and extractions must
+    # count and list exactly what one fresh build does.
+    n = 80
+    vol = torch.from_numpy(workloads.upsampled_noise(n)).cuda()
+    ref = IsoSurface((n, n, n))
+    ref.set_volume(vol)
+    rv, _ = ref.extract_all(0.5)
+    ref.close()
+    iso = IsoSurface((n, n, n))
+    iso.set_volume(vol)
+    for _ in range(4):
+        iso.build(0.5)                     # builds with no extraction in between
+    assert iso.count() == rv.shape[0]
+    out = torch.empty((rv.shape[0], 6), device="cuda")
+    iso.bind_output(out)
+    for _ in range(2):
+        iso.extract()                      # two extractions of one build
+    torch.cuda.synchronize()
+    assert torch.equal(out, rv)
+    iso.build(0.5)
+    iso.build(0.5)
+    out.fill_(float("nan"))
+    iso.extract()
+    torch.cuda.synchronize()
+    assert iso.count() == rv.shape[0] and torch.equal(out, rv)
+    iso.close()
+    # a batch: bare builds, then its graph twice, then an eager extraction
+    specs = workloads.batch_volume_specs(count=4, lo=24, hi=40, balls_lo=3, balls_hi=6, seed=5)
+    vols = [torch.from_numpy(workloads.metaballs(s, b)).cuda() for s, b in specs]
+    b = Batch()
+    for v in vols:
+        b.add_volume(v)
+    b.build(1.0)
+    _, _, total = b.counts()
+    want = torch.empty((total, 6), device="cuda")
+    b.extract(want)
+    torch.cuda.synchronize()
+    g_out = torch.empty_like(want)
+    b.capture_graph(1.0, g_out)
+    for _ in range(3):
+        b.build(1.0)
+    assert b.counts()[2] == total
+    for _ in range(2):
+        g_out.fill_(float("nan"))
+        b.launch_graph()
+        assert b.counts()[2] == total
+        torch.cuda.synchronize()
+        assert torch.equal(g_out, want)
+    again = torch.full_like(want, float("nan"))
+    b.extract(again)
+    torch.cuda.synchronize()
+    assert torch.equal(again, want)
+    b.close()
+
+
 def test_null_volume_after_build_is_a_field_error():
     lattice = (40, 36, 33)
     f = smooth_noise(lattice, 5)
