@@ -2364,12 +2364,13 @@ __device__ __forceinline__ void for_each_vertex(const TraversalView& v, const MK
         const uint64_t* cases, const float4* bnorm, uint2* tile_sums, uint32_t ntiles)        \
     {                                                                                         \
         __shared__ hpmc::EmitShared sh;                                                       \
-        /* the build's scan tile sums, read by its scan only: cleared for the next build */   \
-        if (threadIdx.x == 0)                                                                 \
-            for (uint32_t i = blockIdx.x; i < ntiles; i += gridDim.x) tile_sums[i] = make_uint2(0u, 0u); \
         const uint32_t nitems = hpmc::load_uniform(&totals->items);                           \
         const uint64_t case_word = cases[threadIdx.x];   /* stored to LDS by emit_body */     \
         auto mk = [](const hpmc::VolDesc& dd) { return MAKE_FIELD; };                         \
         hpmc::emit_body<MODE, BATCH>(vols, mk, T, items, l4_off, nitems, out, out_ce,         \
                                      capacity, bnorm, sh, case_word);                         \
+        /* the build's scan tile sums, read by its scan only: cleared for the next build, */  \
+        /* by each workgroup's first thread once its wave is done */                          \
+        if (threadIdx.x == 0)                                                                 \
+            for (uint32_t i = blockIdx.x; i < ntiles; i += gridDim.x) tile_sums[i] = make_uint2(0u, 0u); \
     }
