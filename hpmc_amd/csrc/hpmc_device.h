@@ -1010,13 +1010,17 @@ __device__ __forceinline__ void scan_body(
         const bool one_row = g.tiles_x_log2 >= 4;
         const uint32_t key0 = block_key(g, bxb << g.blog2, by << g.blog2, z0);
         const uint32_t kshift = 2u * (uint32_t)(g.tsx2_log2 - g.blog2);
+        // (the first chunk of each non-empty entry is a plain store; only
+        // blocks of more than kEmitChunk vertices take the chunk loop)
         auto scatter = [&](auto key_of) {
 #pragma unroll
             for (int k = 0; k < kScanPer; ++k) {
-                if (!c[k]) continue;
                 const uint32_t key = key_of(k);
                 const uint32_t yv = (c[k] << 4) | (vol << 16);
-                for (uint32_t ch = 0; ch * kEmitChunk < c[k]; ++ch) items[off.y++] = make_uint2(key, yv | ch);
+                if (c[k]) items[off.y] = make_uint2(key, yv);
+                off.y += c[k] ? 1u : 0u;
+                if (c[k] > (uint32_t)kEmitChunk)
+                    for (uint32_t ch = 1; ch * kEmitChunk < c[k]; ++ch) items[off.y++] = make_uint2(key, yv | ch);
             }
         };
         if (one_row) scatter([&](int k) { return key0 | (morton_spread((uint32_t)k) << kshift); });
