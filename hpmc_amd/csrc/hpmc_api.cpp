@@ -9,6 +9,7 @@
 //   build / count .............................. hpmc/src/hpiface.cpp:225-345
 //   traversal handle and extraction ............ hpmc/src/triface.cpp:43-500
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <hip/hiprtc.h>
 
 #include <cmath>
@@ -406,9 +407,10 @@ struct HPMCHistoPyramid {
     // hipRTC module for snippets
     hipModule_t module = nullptr;
     uint32_t builder_id = 0;
-    // optional per-stage timing: event pairs recorded around each launch;
-    // finished pairs are folded into the totals once a stage holds
-    // kStagePairsMax of them, so profiling left on holds a bounded number
+    // optional per-stage timing: an event pair per launch, recorded by the
+    // launch at the kernel's start and end; finished pairs are folded into
+    // the totals once a stage holds kStagePairsMax of them, so profiling
+    // left on holds a bounded number
     hipEvent_t ev_order = nullptr;   // HPMCsetStream: old stream -> new stream
     bool profiling = false;
     std::vector<hipEvent_t> stage_events[3];
@@ -802,16 +804,26 @@ bool stage_fold(HPMCHistoPyramid* h, int st)
     return good;
 }
 
-void stage_mark(HPMCHistoPyramid* h, int stage)
+// The event pair of a stage's next launch when profiling (else both null):
+// the launch records them at the kernel's start and end.
+void stage_events(HPMCHistoPyramid* h, int stage, hipEvent_t& e0, hipEvent_t& e1)
 {
+    e0 = e1 = nullptr;
     if (!h->profiling) return;
     std::vector<hipEvent_t>& ev = h->stage_events[stage];
-    // an opening mark with the stage full: fold the finished pairs first
-    if (ev.size() % 2 == 0 && ev.size() >= 2 * kStagePairsMax) (void)stage_fold(h, stage);
-    hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return;
-    (void)hipEventRecord(e, h->stream);
-    ev.push_back(e);
+    // the stage full: fold the finished pairs first
+    if (ev.size() >= 2 * kStagePairsMax) (void)stage_fold(h, stage);
+    if (hipEventCreate(&e0) != hipSuccess) {
+        e0 = nullptr;
+        return;
+    }
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        e0 = e1 = nullptr;
+        return;
+    }
+    ev.push_back(e0);
+    ev.push_back(e1);
 }
 
 bool launch_classify(HPMCHistoPyramid* h, float T)
@@ -825,36 +837,36 @@ bool launch_classify(HPMCHistoPyramid* h, float T)
         HIP_TRY(h, hpmc_host::plan_set_field(h->plan, 0, volume_ptr(h), h->stream));
     }
     HIP_TRY(h, hpmc_host::sums_prepare(h->plan, h->stream, false));
-    stage_mark(h, kStageClassify);
+    hipEvent_t e0, e1;
+    stage_events(h, kStageClassify, e0, e1);
     // 8-bit volumes classify raw bytes against the equivalent byte threshold
     const float Tc = h->mode == kFieldVolume && h->unorm8 ? unorm8_threshold(T) : T;
     if (hpmc_host::plan_small_blocks(h->plan, h->plan.vols[0]))
         HIP_TRY(h, hpmc_host::plan_classify_small(h->plan, h->k_classify_small, Tc, h->constants->d_cases,
-                                                  h->stream));
+                                                  h->stream, e0, e1));
     else
-        HIP_TRY(h, hpmc_host::plan_classify(h->plan, h->k_classify, Tc, h->constants->d_cases, h->stream));
+        HIP_TRY(h, hpmc_host::plan_classify(h->plan, h->k_classify, Tc, h->constants->d_cases, h->stream, e0, e1));
     HIP_TRY(h, hipGetLastError());
-    stage_mark(h, kStageClassify);
     return true;
 }
 
 bool launch_scan(HPMCHistoPyramid* h)
 {
-    stage_mark(h, kStageScan);
-    HIP_TRY(h, hpmc_host::plan_scan(h->plan, (const void*)hpmc_scan, h->stream));
+    hipEvent_t e0, e1;
+    stage_events(h, kStageScan, e0, e1);
+    HIP_TRY(h, hpmc_host::plan_scan(h->plan, (const void*)hpmc_scan, h->stream, e0, e1));
     HIP_TRY(h, hipGetLastError());
-    stage_mark(h, kStageScan);
     return true;
 }
 
 bool launch_emit(HPMCHistoPyramid* h, float* out, hpmc::CellEdge* out_ce, uint32_t capacity)
 {
-    stage_mark(h, kStageEmit);
+    hipEvent_t e0, e1;
+    stage_events(h, kStageEmit, e0, e1);
     HIP_TRY(h, hpmc_host::plan_emit(h->plan, h->k_emit, h->threshold, out, out_ce, capacity, h->constants->d_cases,
-                                    h->constants->d_bnorm, h->stream));
+                                    h->constants->d_bnorm, h->stream, e0, e1));
     HIP_TRY(h, hipGetLastError());
     h->plan.sums_dirty = false;   // the extraction clears the tile sums
-    stage_mark(h, kStageEmit);
     return true;
 }
 

@@ -433,13 +433,24 @@ struct Kernel {
     hipFunction_t rtc = nullptr;
 };
 
-inline hipError_t launch(const Kernel& k, unsigned grid, unsigned threads, void** args, hipStream_t stream)
+// A kernel launch.  With events (stage profiling) the kernel's own dispatch
+// timestamps them at its start and end (the extended launch), so no marker
+// packets sit between the kernels and the elapsed time is the kernel's.
+inline hipError_t launch(const Kernel& k, unsigned grid, unsigned threads, void** args, hipStream_t stream,
+                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
-    if (k.rtc) return hipModuleLaunchKernel(k.rtc, grid, 1, 1, threads, 1, 1, 0, stream, args, nullptr);
+    if (k.rtc) {
+        if (e0)
+            return hipExtModuleLaunchKernel(k.rtc, grid * threads, 1, 1, threads, 1, 1, 0, stream, args, nullptr, e0,
+                                            e1, 0);
+        return hipModuleLaunchKernel(k.rtc, grid, 1, 1, threads, 1, 1, 0, stream, args, nullptr);
+    }
+    if (e0) return hipExtLaunchKernel(k.builtin, dim3(grid), dim3(threads), args, 0, stream, e0, e1, 0);
     return hipLaunchKernel(k.builtin, dim3(grid), dim3(threads), args, 0, stream);
 }
 
-inline hipError_t plan_classify(const Plan& p, const Kernel& k, float T, const uint64_t* cases, hipStream_t stream)
+inline hipError_t plan_classify(const Plan& p, const Kernel& k, float T, const uint64_t* cases, hipStream_t stream,
+                                hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     const hpmc::VolDesc* vols = p.d_vols;
     const uint32_t* cv = p.nvol == 1 ? nullptr : p.d_cls_vol;   // one volume: no lookup
@@ -452,13 +463,13 @@ inline hipError_t plan_classify(const Plan& p, const Kernel& k, float T, const u
     void* args[] = {&vols, &cv, &n, &Ta, &cc, &cs, &ts, &nl};
     // the XCD-aware workgroup mapping needs the grid padded to a multiple of 8
     const unsigned grid = (unsigned)(((uint64_t)p.n_wg + 7) / 8 * 8);
-    return launch(k, grid, 64u * (unsigned)p.cls_waves, args, stream);
+    return launch(k, grid, 64u * (unsigned)p.cls_waves, args, stream, e0, e1);
 }
 
 // Classification of a one-volume plan whose blocks are narrower than 16
 // cells (classify_small_body): one thread per column-count entry.
 inline hipError_t plan_classify_small(const Plan& p, const Kernel& k, float T, const uint64_t* cases,
-                                      hipStream_t stream)
+                                      hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     const hpmc::VolDesc* vols = p.d_vols;
     float Ta = T;
@@ -468,7 +479,7 @@ inline hipError_t plan_classify_small(const Plan& p, const Kernel& k, float T, c
     uint32_t nl = p.n_l4t;
     void* args[] = {&vols, &Ta, &cc, &cs, &ts, &nl};
     const unsigned grid = (unsigned)((p.n_cc + 255) / 256);
-    return launch(k, grid, 256, args, stream);
+    return launch(k, grid, 256, args, stream, e0, e1);
 }
 
 // True when every volume's cell indices fit the 32-bit cell field of the
@@ -486,7 +497,8 @@ inline bool plan_small_blocks(const Plan& p, const hpmc::VolDesc& d)
     return p.nvol == 1 && d.g.blog2 < hpmc::kBlockLog2;
 }
 
-inline hipError_t plan_scan(const Plan& p, const void* k_scan, hipStream_t stream)
+inline hipError_t plan_scan(const Plan& p, const void* k_scan, hipStream_t stream, hipEvent_t e0 = nullptr,
+                            hipEvent_t e1 = nullptr)
 {
     const unsigned grid = p.n_l4t + p.n_colt;
     const hpmc::VolDesc* vols = p.d_vols;
@@ -501,11 +513,12 @@ inline hipError_t plan_scan(const Plan& p, const void* k_scan, hipStream_t strea
     hpmc::VolCount* dv = p.d_vcounts;
     hpmc::VolCount* hv = p.h_vcounts;
     void* args[] = {&vols, &tv, &nl, &nc, &cc, &ts, &lo, &it, &dt, &ht, &dv, &hv};
-    return hipLaunchKernel(k_scan, dim3(grid), dim3(hpmc::kScanThreads), args, 0, stream);
+    return launch(Kernel{k_scan, nullptr}, grid, hpmc::kScanThreads, args, stream, e0, e1);
 }
 
 inline hipError_t plan_emit(const Plan& p, const Kernel& k, float T, float* out, hpmc::CellEdge* out_ce,
-                            uint32_t capacity, const uint64_t* cases, const float4* bnorm, hipStream_t stream)
+                            uint32_t capacity, const uint64_t* cases, const float4* bnorm, hipStream_t stream,
+                            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     const hpmc::VolDesc* vols = p.d_vols;
     float Ta = T;
@@ -520,7 +533,7 @@ inline hipError_t plan_emit(const Plan& p, const Kernel& k, float T, float* out,
     uint2* ts = p.tile_sums;
     uint32_t nts = p.n_l4t + p.n_colt;
     void* args[] = {&vols, &Ta, &it, &lo, &tt, &o, &oc, &cap, &cs, &bn, &ts, &nts};
-    return launch(k, (unsigned)p.emit_wgs, 256, args, stream);
+    return launch(k, (unsigned)p.emit_wgs, 256, args, stream, e0, e1);
 }
 
 }  // namespace hpmc_host

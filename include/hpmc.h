@@ -237,8 +237,10 @@ int32_t HPMCgetCellCodes(struct HPMCHistoPyramid* h, uint8_t* device_codes);
 uint64_t HPMCgetLevel4(struct HPMCHistoPyramid* h, uint32_t* device_out, uint64_t capacity);
 
 /* Per-stage timing with HIP events on the HistoPyramid's stream: when
- * enabled, every launch of classify, scan and emit is bracketed by a pair of
- * events.  HPMCgetStageTimes waits for them, writes the summed milliseconds
+ * enabled, every launch of classify, scan and emit records a pair of events
+ * at its kernel's start and end (hipExtLaunchKernel), so the times are the
+ * kernels' own; the launches are spaced further apart meanwhile.
+ * HPMCgetStageTimes waits for them, writes the summed milliseconds
  * (ms[3] = classify, scan, emit) and launch counts, and starts over.  While
  * profiling stays on without a HPMCgetStageTimes call, every 1024 pairs of a
  * stage are folded into its totals (a wait on the device), so the events
