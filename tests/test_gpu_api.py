@@ -184,6 +184,34 @@ def test_profiling_events_stay_bounded_and_count_every_launch():
     iso.close()
 
 
+def test_stage_times_of_a_snippet_field():
+    # the hipRTC kernels of a snippet field are timed through their own
+    # launches too (the module launch with events)
+    lattice = (40, 36, 33)
+    iso = IsoSurface(lattice)
+    iso.set_snippet(workloads.CAYLEY_SNIPPET)
+    iso.build(0.0)
+    n = iso.count()
+    assert n > 0
+    out = torch.empty((n, 6), device="cuda")
+    iso.bind_output(out)
+    iso.stage_times()
+    iso.set_profiling(True)
+    for _ in range(3):
+        iso.build(0.0)
+        iso.extract()
+    iso.set_profiling(False)
+    st = iso.stage_times()
+    assert all(st[k][1] == 3 and st[k][0] > 0 for k in ("classify", "scan", "emit")), st
+    ref = torch.empty_like(out)
+    iso.bind_output(ref)
+    iso.build(0.0)
+    iso.extract()
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)          # profiled launches compute the same
+    iso.close()
+
+
 def test_graph_replay_extracts_at_the_captured_threshold():
     specs = workloads.batch_volume_specs(count=6, lo=16, hi=48, balls_lo=4, balls_hi=8, seed=11)
     vols = [torch.from_numpy(workloads.metaballs(s, b)).cuda() for s, b in specs]
