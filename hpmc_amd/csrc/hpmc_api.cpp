@@ -49,11 +49,11 @@ HPMC_DEFINE_EMIT_KERNEL(hpmc_emit_volume_batch, HPMC_VOLUME_FIELD, hpmc::kEmitCo
 
 extern "C" __global__ __launch_bounds__(hpmc::kScanThreads) void hpmc_scan(
     const hpmc::VolDesc* vols, const uint32_t* tile_vol, uint32_t n_l4t, uint32_t n_colt, const uint16_t* colcnt,
-    const uint2* tile_sums, uint32_t* l4_off, uint2* items, hpmc::Totals* totals, hpmc::Totals* host_totals,
-    hpmc::VolCount* vol_counts, hpmc::VolCount* host_vol_counts)
+    const uint2* tile_sums, uint32_t* item_reps, uint32_t* l4_off, uint2* items, hpmc::Totals* totals,
+    hpmc::Totals* host_totals, hpmc::VolCount* vol_counts, hpmc::VolCount* host_vol_counts)
 {
-    hpmc::scan_body(vols, tile_vol, n_l4t, n_colt, colcnt, tile_sums, l4_off, items, totals, host_totals,
-                    vol_counts, host_vol_counts);
+    hpmc::scan_body(vols, tile_vol, n_l4t, n_colt, colcnt, tile_sums, item_reps, l4_off, items, totals,
+                    host_totals, vol_counts, host_vol_counts);
 }
 
 extern "C" __global__ __launch_bounds__(256) void hpmc_level4_gather(const hpmc::VolDesc* vols, uint32_t vol,

@@ -176,6 +176,7 @@ struct Plan {
     uint2* tile_sums = nullptr;
     uint2* items = nullptr;
     hpmc::Totals* d_totals = nullptr;
+    uint32_t* d_item_reps = nullptr;      // copies of the item total for emission
     hpmc::Totals* h_totals = nullptr;     // pinned
     hpmc::VolCount* d_vcounts = nullptr;
     hpmc::VolCount* h_vcounts = nullptr;  // pinned
@@ -197,6 +198,7 @@ inline void plan_free(Plan& p)
     if (p.tile_sums) (void)hipFree(p.tile_sums);
     if (p.items) (void)hipFree(p.items);
     if (p.d_totals) (void)hipFree(p.d_totals);
+    if (p.d_item_reps) (void)hipFree(p.d_item_reps);
     if (p.h_totals) (void)hipHostFree(p.h_totals);
     if (p.d_vcounts) (void)hipFree(p.d_vcounts);
     if (p.h_vcounts) (void)hipHostFree(p.h_vcounts);
@@ -372,6 +374,8 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
     PLAN_TRY(hipMalloc(&p.items, std::max<uint64_t>(p.n_items_max, 1) * sizeof(uint2)));
     PLAN_TRY(hipMalloc(&p.d_totals, sizeof(hpmc::Totals)));
     PLAN_TRY(hipMemset(p.d_totals, 0, sizeof(hpmc::Totals)));
+    PLAN_TRY(hipMalloc(&p.d_item_reps, hpmc::kItemReps * hpmc::kItemRepStride * sizeof(uint32_t)));
+    PLAN_TRY(hipMemset(p.d_item_reps, 0, hpmc::kItemReps * hpmc::kItemRepStride * sizeof(uint32_t)));
     PLAN_TRY(hipHostMalloc(&p.h_totals, sizeof(hpmc::Totals), hipHostMallocDefault));
     memset(p.h_totals, 0, sizeof(hpmc::Totals));
     // adjoint gather tiles of each volume
@@ -398,7 +402,7 @@ inline int32_t plan_create(Plan& p, const std::vector<VolumeSpec>& specs, const 
 }
 
 // Device bytes of a one-volume plan's arrays carved from one buffer (each
-// array 256-byte aligned): descriptor, totals, volume count,
+// array 256-byte aligned): descriptor, totals, item total copies, volume count,
 // tile sums, column counts, block offsets, work items.
 inline uint64_t plan_carve(Plan& p, char* base)
 {
@@ -410,6 +414,7 @@ inline uint64_t plan_carve(Plan& p, char* base)
     };
     p.d_vols = (hpmc::VolDesc*)take(sizeof(hpmc::VolDesc));
     p.d_totals = (hpmc::Totals*)take(sizeof(hpmc::Totals));
+    p.d_item_reps = (uint32_t*)take(hpmc::kItemReps * hpmc::kItemRepStride * sizeof(uint32_t));
     p.d_vcounts = (hpmc::VolCount*)take(sizeof(hpmc::VolCount));
     p.tile_sums = (uint2*)take((uint64_t)(p.n_l4t + p.n_colt) * sizeof(uint2));
     p.colcnt = (uint16_t*)take(p.n_cc * sizeof(uint16_t));
@@ -506,13 +511,14 @@ inline hipError_t plan_scan(const Plan& p, const void* k_scan, hipStream_t strea
     uint32_t nl = p.n_l4t, nc = p.n_colt;
     const uint16_t* cc = p.colcnt;
     const uint2* ts = p.tile_sums;
+    uint32_t* ir = p.d_item_reps;
     uint32_t* lo = p.l4_off;
     uint2* it = p.items;
     hpmc::Totals* dt = p.d_totals;
     hpmc::Totals* ht = p.h_totals;
     hpmc::VolCount* dv = p.d_vcounts;
     hpmc::VolCount* hv = p.h_vcounts;
-    void* args[] = {&vols, &tv, &nl, &nc, &cc, &ts, &lo, &it, &dt, &ht, &dv, &hv};
+    void* args[] = {&vols, &tv, &nl, &nc, &cc, &ts, &ir, &lo, &it, &dt, &ht, &dv, &hv};
     return launch(Kernel{k_scan, nullptr}, grid, hpmc::kScanThreads, args, stream, e0, e1);
 }
 
@@ -524,7 +530,7 @@ inline hipError_t plan_emit(const Plan& p, const Kernel& k, float T, float* out,
     float Ta = T;
     const uint2* it = p.items;
     const uint32_t* lo = p.l4_off;
-    const hpmc::Totals* tt = p.d_totals;
+    const uint32_t* tt = p.d_item_reps;
     float* o = out;
     hpmc::CellEdge* oc = out_ce;
     uint32_t cap = capacity;
